@@ -1,0 +1,612 @@
+// Host side: geometry plans (per-view FP64 tables on the device), the
+// device-resident plan API, and CTLAB's host-pointer entry points
+// DD3Proj_gpu / DD3Back_gpu / DD2_multiGPU as thin H2D -> plan -> D2H wrappers.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/ctlab_mi355x.h"
+#include "dd_common.hpp"
+#include "status.hpp"
+
+namespace ctlab {
+
+template <typename T>
+hipError_t launch_proj(const DevGeom& g, const float* d_vol, void* d_cz, float* d_prj, hipStream_t s);
+template <typename T>
+hipError_t launch_back(const DevGeom& g, const float* d_prj, void* d_P, float* d_vol,
+                       const int* d_views_y, int ny, const int* d_views_x, int nx, hipStream_t s);
+
+// ---------------------------------------------------------------------------
+// status (per host thread)
+// ---------------------------------------------------------------------------
+static thread_local int t_err = CTLAB_OK;
+static thread_local char t_msg[512] = "";
+
+void set_error(int code, const char* fmt, ...)
+{
+    t_err = code;
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(t_msg, sizeof(t_msg), fmt, ap);
+    va_end(ap);
+}
+
+void clear_error()
+{
+    t_err = CTLAB_OK;
+    t_msg[0] = 0;
+}
+
+int hip_check(hipError_t e, const char* what)
+{
+    if (e == hipSuccess) return 0;
+    set_error(e == hipErrorOutOfMemory ? CTLAB_ERR_NOMEM : CTLAB_ERR_HIP, "%s: %s", what,
+              hipGetErrorString(e));
+    return -1;
+}
+
+// src/utilities.cuh:413-433 (DD3Boundaries)
+static void boundaries(int nb, const double* c, double* b)
+{
+    if (nb >= 3) {
+        b[0] = 1.5 * c[0] - 0.5 * c[1];
+        for (int i = 1; i <= nb - 2; ++i) b[i] = 0.5 * c[i - 1] + 0.5 * c[i];
+        b[nb - 1] = 1.5 * c[nb - 2] - 0.5 * c[nb - 3];
+    } else {
+        b[0] = c[0] - 0.5;
+        b[1] = c[0] + 0.5;
+    }
+}
+
+// src/utilities.cuh:279-286 (regularizeAngle)
+static double regularize_angle(double a)
+{
+    const double twopi = 6.283185307179586;
+    while (a >= twopi) a -= twopi;
+    while (a < 0) a += twopi;
+    return a;
+}
+
+struct GeomIn {
+    double x0, y0, z0;
+    int DNU, DNV;
+    std::vector<double> xds, yds, zds, angs, zpos;
+    double cx, cy, cz;
+    int PN, XN, YN, ZN;
+    double dx, dz;
+    std::vector<unsigned char> mask;  // empty = none
+    int cone;
+};
+
+}  // namespace ctlab
+
+struct ctlab_plan {
+    int device = 0;
+    ctlab::GeomIn in;
+    std::vector<ctlab::ViewParams> vp;
+    std::vector<double> kap, tau, colfac, exy2, zhat, ez, rowfac;
+    double zhat0 = 0, dzh = 1;
+    std::vector<int> views_y, views_x;
+    // device copies
+    ctlab::ViewParams* d_vp = nullptr;
+    double *d_kap = nullptr, *d_tau = nullptr, *d_colfac = nullptr, *d_exy2 = nullptr;
+    double *d_zhat = nullptr, *d_ez = nullptr, *d_rowfac = nullptr;
+    unsigned char* d_mask = nullptr;
+    int* d_views = nullptr;
+    void* d_cz = nullptr;   // z-prefix integral image (nvox elements)
+    void* d_P = nullptr;    // projection v-prefix (nprj elements)
+    int sat_bits = 64;
+    size_t ws_elem = 0;
+
+    ctlab::DevGeom dev() const
+    {
+        ctlab::DevGeom g;
+        g.XN = in.XN; g.YN = in.YN; g.ZN = in.ZN; g.DNU = in.DNU; g.DNV = in.DNV; g.PN = in.PN;
+        g.dx = in.dx; g.dz = in.dz; g.cone = in.cone;
+        g.vp = d_vp; g.kap = d_kap; g.tau = d_tau; g.colfac = d_colfac; g.exy2 = d_exy2;
+        g.zhat = d_zhat; g.ez = d_ez; g.rowfac = d_rowfac;
+        g.mask = in.mask.empty() ? nullptr : d_mask;
+        g.zhat0 = zhat0; g.dzh = dzh;
+        return g;
+    }
+};
+
+namespace ctlab {
+
+static bool finite_all(const std::vector<double>& v)
+{
+    for (double x : v)
+        if (!std::isfinite(x)) return false;
+    return true;
+}
+
+// FP64 per-view tables; same formulas as the CPU oracle (dd_oracle.c).
+static int build_tables(ctlab_plan* P)
+{
+    const GeomIn& g = P->in;
+    if (g.DNU <= 0 || g.DNV <= 0 || g.PN <= 0 || g.XN <= 0 || g.YN <= 0 || g.ZN <= 0) {
+        set_error(CTLAB_ERR_ARG, "sizes must be positive (DNU=%d DNV=%d PN=%d XN=%d YN=%d ZN=%d)",
+                  g.DNU, g.DNV, g.PN, g.XN, g.YN, g.ZN);
+        return -1;
+    }
+    if (!(g.dx > 0) || !(g.dz > 0) || !std::isfinite(g.dx) || !std::isfinite(g.dz)) {
+        set_error(CTLAB_ERR_ARG, "voxel pitches must be positive and finite");
+        return -1;
+    }
+    if (!std::isfinite(g.x0) || !std::isfinite(g.y0) || !std::isfinite(g.z0) || !std::isfinite(g.cx) ||
+        !std::isfinite(g.cy) || !std::isfinite(g.cz) || !finite_all(g.xds) || !finite_all(g.yds) ||
+        !finite_all(g.zds) || !finite_all(g.angs) || !finite_all(g.zpos)) {
+        set_error(CTLAB_ERR_ARG, "non-finite geometry");
+        return -1;
+    }
+    if (g.ZN > 64 || g.DNV > 64) {
+        set_error(CTLAB_ERR_GEOMETRY, "ZN (%d) and DNV (%d) must be <= 64", g.ZN, g.DNV);
+        return -1;
+    }
+    if (!g.cone && g.ZN != g.DNV) {
+        set_error(CTLAB_ERR_ARG, "multi-slice mode needs ZN == DNV");
+        return -1;
+    }
+    const int DNU = g.DNU, DNV = g.DNV, PN = g.PN;
+    std::vector<double> xb(DNU + 1), yb(DNU + 1), zb(DNV + 1);
+    boundaries(DNU + 1, g.xds.data(), xb.data());
+    boundaries(DNU + 1, g.yds.data(), yb.data());
+    if (g.cone) boundaries(DNV + 1, g.zds.data(), zb.data());
+
+    P->zhat.resize(DNV + 1);
+    P->ez.resize(DNV);
+    P->rowfac.resize(DNV);
+    for (int v = 0; v <= DNV; ++v) P->zhat[v] = g.cone ? (zb[v] - g.z0) / g.dz : (double)v;
+    for (int v = 0; v < DNV; ++v) {
+        P->ez[v] = g.cone ? (g.zds[v] - g.z0) : 0.0;
+        const double h = P->zhat[v + 1] - P->zhat[v];
+        if (h == 0.0) {
+            set_error(CTLAB_ERR_GEOMETRY, "zero-height detector row %d", v);
+            return -1;
+        }
+        P->rowfac[v] = g.cone ? 1.0 / h : 1.0;
+    }
+    // uniform row model for the backprojector (rows must be evenly spaced)
+    P->zhat0 = P->zhat[0];
+    P->dzh = (P->zhat[DNV] - P->zhat[0]) / DNV;
+    for (int v = 0; v <= DNV; ++v) {
+        const double dev = std::fabs(P->zhat[v] - (P->zhat0 + v * P->dzh));
+        if (dev > 1e-4 * std::fabs(P->dzh)) {
+            set_error(CTLAB_ERR_GEOMETRY, "detector rows are not evenly spaced (row %d off by %g rows)",
+                      v, dev / std::fabs(P->dzh));
+            return -1;
+        }
+    }
+
+    const double Xmin = g.cx - g.XN * g.dx * 0.5;
+    const double Ymin = g.cy - g.YN * g.dx * 0.5;
+    const double Zmin = g.cz - g.ZN * g.dz * 0.5;
+    P->vp.resize(PN);
+    P->kap.resize((size_t)PN * (DNU + 1));
+    P->tau.resize((size_t)PN * DNU);
+    P->colfac.resize((size_t)PN * DNU);
+    P->exy2.resize((size_t)PN * DNU);
+    P->views_y.clear();
+    P->views_x.clear();
+    for (int p = 0; p < PN; ++p) {
+        const double th = regularize_angle(g.angs[p]);
+        const double c = std::cos(th), s = std::sin(th);
+        const double Sx = g.x0 * c - g.y0 * s, Sy = g.x0 * s + g.y0 * c;
+        const double zs = g.z0 + g.zpos[p];
+        const int yslab = std::fabs(g.cy - Sy) >= std::fabs(g.cx - Sx);
+        ViewParams& V = P->vp[p];
+        V.yslab = yslab;
+        double Ss, Sr, Smin, Rmin;
+        if (yslab) {
+            V.NS = g.YN; V.NR = g.XN; V.slab_stride = g.XN; V.row_stride = 1;
+            Ss = Sy; Sr = Sx; Smin = Ymin; Rmin = Xmin;
+        } else {
+            V.NS = g.XN; V.NR = g.YN; V.slab_stride = 1; V.row_stride = g.XN;
+            Ss = Sx; Sr = Sy; Smin = Xmin; Rmin = Ymin;
+        }
+        V.s0 = Smin + 0.5 * g.dx - Ss;
+        V.rbase = (Sr - Rmin) / g.dx;
+        V.beta0 = g.cone ? (zs - Zmin) / g.dz : 0.0;
+        const double slast = V.s0 + (V.NS - 1) * g.dx;
+        if (!(V.s0 * slast > 0)) {
+            set_error(CTLAB_ERR_GEOMETRY, "view %d: source lies inside the volume's slab range", p);
+            return -1;
+        }
+        double* kp = &P->kap[(size_t)p * (DNU + 1)];
+        for (int u = 0; u <= DNU; ++u) {
+            const double X = xb[u] * c - yb[u] * s, Y = xb[u] * s + yb[u] * c;
+            const double ds = yslab ? Y : X, dr = yslab ? X : Y;
+            kp[u] = (dr - Sr) / (ds - Ss);
+        }
+        for (int u = 0; u <= DNU; ++u) {
+            if (!std::isfinite(kp[u])) {
+                set_error(CTLAB_ERR_GEOMETRY, "view %d: detector boundary %d is level with the source", p, u);
+                return -1;
+            }
+        }
+        const double dk0 = kp[1] - kp[0];
+        for (int u = 0; u < DNU; ++u) {
+            const double dk = kp[u + 1] - kp[u];
+            if (!(dk * dk0 > 0)) {
+                set_error(CTLAB_ERR_GEOMETRY, "view %d: detector boundaries are not monotone", p);
+                return -1;
+            }
+            const double X = g.xds[u] * c - g.yds[u] * s, Y = g.xds[u] * s + g.yds[u] * c;
+            const double ex = X - Sx, ey = Y - Sy;
+            const double es = yslab ? ey : ex;
+            if (es == 0.0) {
+                set_error(CTLAB_ERR_GEOMETRY, "view %d: detector cell %d is level with the source", p, u);
+                return -1;
+            }
+            const double t = 1.0 / es;
+            P->tau[(size_t)p * DNU + u] = t;
+            P->exy2[(size_t)p * DNU + u] = ex * ex + ey * ey;
+            P->colfac[(size_t)p * DNU + u] =
+                g.cone ? g.dx * g.dx / std::fabs(dk) : g.dx * g.dx * std::fabs(t) / std::fabs(dk);
+        }
+        V.asc = (V.s0 * dk0) > 0;
+        (yslab ? P->views_y : P->views_x).push_back(p);
+    }
+    return 0;
+}
+
+template <typename T>
+static int upload(T** dst, const std::vector<T>& src)
+{
+    if (src.empty()) { *dst = nullptr; return 0; }
+    if (hip_check(hipMalloc((void**)dst, src.size() * sizeof(T)), "hipMalloc")) return -1;
+    return hip_check(hipMemcpy(*dst, src.data(), src.size() * sizeof(T), hipMemcpyHostToDevice),
+                     "hipMemcpy");
+}
+
+static void free_plan(ctlab_plan* P)
+{
+    if (!P) return;
+    int cur = 0;
+    hipGetDevice(&cur);
+    hipSetDevice(P->device);
+    void* ptrs[] = {P->d_vp, P->d_kap, P->d_tau, P->d_colfac, P->d_exy2, P->d_zhat, P->d_ez,
+                    P->d_rowfac, P->d_mask, P->d_views, P->d_cz, P->d_P};
+    for (void* q : ptrs)
+        if (q) hipFree(q);
+    hipSetDevice(cur);
+    delete P;
+}
+
+static int alloc_workspace(ctlab_plan* P)
+{
+    const size_t es = P->sat_bits == 32 ? 4 : 8;
+    const size_t nvox = (size_t)P->in.XN * P->in.YN * P->in.ZN;
+    const size_t nprj = (size_t)P->in.PN * P->in.DNU * P->in.DNV;
+    if (P->d_cz) hipFree(P->d_cz);
+    if (P->d_P) hipFree(P->d_P);
+    P->d_cz = P->d_P = nullptr;
+    if (hip_check(hipMalloc(&P->d_cz, nvox * es), "hipMalloc(integral image)")) return -1;
+    if (hip_check(hipMalloc(&P->d_P, nprj * es), "hipMalloc(projection prefix)")) return -1;
+    P->ws_elem = es;
+    return 0;
+}
+
+static ctlab_plan* finish_plan(ctlab_plan* P, int gpunum)
+{
+    if (build_tables(P)) { delete P; return nullptr; }
+    int ndev = 0;
+    if (hip_check(hipGetDeviceCount(&ndev), "hipGetDeviceCount")) { delete P; return nullptr; }
+    if (gpunum < 0 || gpunum >= ndev) {
+        set_error(CTLAB_ERR_ARG, "gpunum %d out of range (%d devices)", gpunum, ndev);
+        delete P;
+        return nullptr;
+    }
+    P->device = gpunum;
+    if (hip_check(hipSetDevice(gpunum), "hipSetDevice")) { delete P; return nullptr; }
+    std::vector<int> views(P->views_y);
+    views.insert(views.end(), P->views_x.begin(), P->views_x.end());
+    if (upload(&P->d_vp, P->vp) || upload(&P->d_kap, P->kap) || upload(&P->d_tau, P->tau) ||
+        upload(&P->d_colfac, P->colfac) || upload(&P->d_exy2, P->exy2) || upload(&P->d_zhat, P->zhat) ||
+        upload(&P->d_ez, P->ez) || upload(&P->d_rowfac, P->rowfac) || upload(&P->d_mask, P->in.mask) ||
+        upload(&P->d_views, views) || alloc_workspace(P)) {
+        free_plan(P);
+        return nullptr;
+    }
+    return P;
+}
+
+static std::vector<double> to_d(const float* p, int n)
+{
+    std::vector<double> v(n);
+    for (int i = 0; i < n; ++i) v[i] = p[i];
+    return v;
+}
+
+ctlab_plan* plan_create_cone(float x0, float y0, float z0, int DNU, int DNV, const float* xds,
+                             const float* yds, const float* zds, float imgXCenter, float imgYCenter,
+                             float imgZCenter, const float* hangs, const float* hzPos, int PN, int XN,
+                             int YN, int ZN, float dx, float dz, const byte* mask, int gpunum)
+{
+    if (!xds || !yds || !zds || !hangs || !hzPos) {
+        set_error(CTLAB_ERR_ARG, "NULL geometry array");
+        return nullptr;
+    }
+    if (DNU <= 0 || DNV <= 0 || PN <= 0 || XN <= 0 || YN <= 0 || ZN <= 0) {
+        set_error(CTLAB_ERR_ARG, "sizes must be positive");
+        return nullptr;
+    }
+    ctlab_plan* P = new ctlab_plan();
+    GeomIn& g = P->in;
+    g.x0 = x0; g.y0 = y0; g.z0 = z0; g.DNU = DNU; g.DNV = DNV;
+    g.xds = to_d(xds, DNU); g.yds = to_d(yds, DNU); g.zds = to_d(zds, DNV);
+    g.angs = to_d(hangs, PN); g.zpos = to_d(hzPos, PN);
+    g.cx = imgXCenter; g.cy = imgYCenter; g.cz = imgZCenter;
+    g.PN = PN; g.XN = XN; g.YN = YN; g.ZN = ZN; g.dx = dx; g.dz = dz; g.cone = 1;
+    if (mask) g.mask.assign(mask, mask + (size_t)XN * YN);
+    return finish_plan(P, gpunum);
+}
+
+ctlab_plan* plan_create_fan(float x0, float y0, const float* xds, const float* yds, int DNU, int SLN,
+                            float imgXCenter, float imgYCenter, int XN, int YN, float dx,
+                            const float* hangs, int PN, const byte* mask, int gpunum)
+{
+    if (!xds || !yds || !hangs) {
+        set_error(CTLAB_ERR_ARG, "NULL geometry array");
+        return nullptr;
+    }
+    if (DNU <= 0 || SLN <= 0 || PN <= 0 || XN <= 0 || YN <= 0) {
+        set_error(CTLAB_ERR_ARG, "sizes must be positive");
+        return nullptr;
+    }
+    ctlab_plan* P = new ctlab_plan();
+    GeomIn& g = P->in;
+    g.x0 = x0; g.y0 = y0; g.z0 = 0; g.DNU = DNU; g.DNV = SLN;
+    g.xds = to_d(xds, DNU); g.yds = to_d(yds, DNU); g.zds.assign(SLN, 0.0);
+    g.angs = to_d(hangs, PN); g.zpos.assign(PN, 0.0);
+    g.cx = imgXCenter; g.cy = imgYCenter; g.cz = 0;
+    g.PN = PN; g.XN = XN; g.YN = YN; g.ZN = SLN; g.dx = dx; g.dz = 1.0; g.cone = 0;
+    if (mask) g.mask.assign(mask, mask + (size_t)XN * YN);
+    return finish_plan(P, gpunum);
+}
+
+struct DeviceGuard {
+    int prev = -1;
+    explicit DeviceGuard(int dev)
+    {
+        hipGetDevice(&prev);
+        if (prev != dev) hipSetDevice(dev); else prev = -1;
+    }
+    ~DeviceGuard()
+    {
+        if (prev >= 0) hipSetDevice(prev);
+    }
+};
+
+int plan_proj(ctlab_plan* P, const float* d_vol, float* d_prj, hipStream_t s)
+{
+    if (!P || !d_vol || !d_prj) {
+        set_error(CTLAB_ERR_ARG, "NULL plan or buffer");
+        return CTLAB_ERR_ARG;
+    }
+    DeviceGuard dg(P->device);
+    const DevGeom g = P->dev();
+    hipError_t e = P->sat_bits == 32 ? launch_proj<float>(g, d_vol, P->d_cz, d_prj, s)
+                                     : launch_proj<double>(g, d_vol, P->d_cz, d_prj, s);
+    if (hip_check(e, "projector launch")) return t_err;
+    return CTLAB_OK;
+}
+
+int plan_back(ctlab_plan* P, const float* d_prj, float* d_vol, hipStream_t s)
+{
+    if (!P || !d_vol || !d_prj) {
+        set_error(CTLAB_ERR_ARG, "NULL plan or buffer");
+        return CTLAB_ERR_ARG;
+    }
+    DeviceGuard dg(P->device);
+    const DevGeom g = P->dev();
+    const int ny = (int)P->views_y.size(), nx = (int)P->views_x.size();
+    const int* vy = P->d_views;
+    const int* vx = P->d_views + ny;
+    hipError_t e = P->sat_bits == 32 ? launch_back<float>(g, d_prj, P->d_P, d_vol, vy, ny, vx, nx, s)
+                                     : launch_back<double>(g, d_prj, P->d_P, d_vol, vy, ny, vx, nx, s);
+    if (hip_check(e, "backprojector launch")) return t_err;
+    return CTLAB_OK;
+}
+
+// host-pointer round trip used by the CTLAB entry points
+static void host_call(ctlab_plan* P, float* hvol, float* hprj, bool forward)
+{
+    if (!P) return;
+    DeviceGuard dg(P->device);
+    const size_t nvox = (size_t)P->in.XN * P->in.YN * P->in.ZN;
+    const size_t nprj = (size_t)P->in.PN * P->in.DNU * P->in.DNV;
+    float *dv = nullptr, *dp = nullptr;
+    hipStream_t s = nullptr;
+    bool ok = !hip_check(hipStreamCreate(&s), "hipStreamCreate") &&
+              !hip_check(hipMalloc(&dv, nvox * sizeof(float)), "hipMalloc(volume)") &&
+              !hip_check(hipMalloc(&dp, nprj * sizeof(float)), "hipMalloc(projection)");
+    if (ok) {
+        if (forward) {
+            ok = !hip_check(hipMemcpyAsync(dv, hvol, nvox * sizeof(float), hipMemcpyHostToDevice, s), "H2D") &&
+                 plan_proj(P, dv, dp, s) == CTLAB_OK;
+        } else {
+            ok = !hip_check(hipMemcpyAsync(dp, hprj, nprj * sizeof(float), hipMemcpyHostToDevice, s), "H2D") &&
+                 plan_back(P, dp, dv, s) == CTLAB_OK;
+        }
+        if (ok) ok = !hip_check(hipStreamSynchronize(s), "kernel execution");
+        if (ok) {
+            ok = forward ? !hip_check(hipMemcpy(hprj, dp, nprj * sizeof(float), hipMemcpyDeviceToHost), "D2H")
+                         : !hip_check(hipMemcpy(hvol, dv, nvox * sizeof(float), hipMemcpyDeviceToHost), "D2H");
+        }
+    }
+    if (dv) hipFree(dv);
+    if (dp) hipFree(dp);
+    if (s) hipStreamDestroy(s);
+}
+
+}  // namespace ctlab
+
+using namespace ctlab;
+
+extern "C" {
+
+int ctlab_last_error(void) { return ctlab::t_err; }
+const char* ctlab_last_error_message(void) { return ctlab::t_msg; }
+
+const char* ctlab_error_string(int code)
+{
+    switch (code) {
+    case CTLAB_OK: return "success";
+    case CTLAB_ERR_ARG: return "invalid argument";
+    case CTLAB_ERR_MODE: return "unsupported projection mode";
+    case CTLAB_ERR_GEOMETRY: return "unsupported geometry";
+    case CTLAB_ERR_HIP: return "HIP runtime error";
+    case CTLAB_ERR_NOMEM: return "device allocation failed";
+    default: return "unknown error";
+    }
+}
+
+ctlab_plan* ctlab_plan_create(float x0, float y0, float z0, int DNU, int DNV, const float* xds,
+                              const float* yds, const float* zds, float imgXCenter, float imgYCenter,
+                              float imgZCenter, const float* hangs, const float* hzPos, int PN, int XN,
+                              int YN, int ZN, float dx, float dz, const byte* mask, int gpunum)
+{
+    clear_error();
+    return plan_create_cone(x0, y0, z0, DNU, DNV, xds, yds, zds, imgXCenter, imgYCenter, imgZCenter,
+                            hangs, hzPos, PN, XN, YN, ZN, dx, dz, mask, gpunum);
+}
+
+ctlab_plan* ctlab_plan_create_fan(float x0, float y0, const float* xds, const float* yds, int DNU,
+                                  int SLN, float imgXCenter, float imgYCenter, int XN, int YN, float dx,
+                                  const float* hangs, int PN, const byte* mask, int gpunum)
+{
+    clear_error();
+    return plan_create_fan(x0, y0, xds, yds, DNU, SLN, imgXCenter, imgYCenter, XN, YN, dx, hangs, PN,
+                           mask, gpunum);
+}
+
+void ctlab_plan_destroy(ctlab_plan* plan) { free_plan(plan); }
+
+int ctlab_plan_set_sat_bits(ctlab_plan* plan, int bits)
+{
+    clear_error();
+    if (!plan || (bits != 32 && bits != 64)) {
+        set_error(CTLAB_ERR_ARG, "sat bits must be 32 or 64");
+        return CTLAB_ERR_ARG;
+    }
+    if (plan->sat_bits == bits) return CTLAB_OK;
+    DeviceGuard dg(plan->device);
+    plan->sat_bits = bits;
+    if (alloc_workspace(plan)) return t_err;
+    return CTLAB_OK;
+}
+
+int ctlab_proj(ctlab_plan* plan, const float* d_vol, float* d_prj, void* stream)
+{
+    clear_error();
+    return plan_proj(plan, d_vol, d_prj, (hipStream_t)stream);
+}
+
+int ctlab_back(ctlab_plan* plan, const float* d_prj, float* d_vol, void* stream)
+{
+    clear_error();
+    return plan_back(plan, d_prj, d_vol, (hipStream_t)stream);
+}
+
+int ctlab_plan_views(const ctlab_plan* plan, int* n_yslab, int* n_xslab)
+{
+    if (!plan) return CTLAB_ERR_ARG;
+    if (n_yslab) *n_yslab = (int)plan->views_y.size();
+    if (n_xslab) *n_xslab = (int)plan->views_x.size();
+    return CTLAB_OK;
+}
+
+long long ctlab_plan_workspace_bytes(const ctlab_plan* plan)
+{
+    if (!plan) return -1;
+    const size_t nvox = (size_t)plan->in.XN * plan->in.YN * plan->in.ZN;
+    const size_t nprj = (size_t)plan->in.PN * plan->in.DNU * plan->in.DNV;
+    return (long long)((nvox + nprj) * plan->ws_elem);
+}
+
+void DD3Proj_gpu(float x0, float y0, float z0, int DNU, int DNV, float* xds, float* yds, float* zds,
+                 float imgXCenter, float imgYCenter, float imgZCenter, float* hangs, float* hzPos, int PN,
+                 int XN, int YN, int ZN, float* hvol, float* hprj, float dx, float dz, byte* mask,
+                 int gpunum, int prjMode)
+{
+    clear_error();
+    if (prjMode != 0) {
+        set_error(CTLAB_ERR_MODE, "prjMode %d not supported (branchless DD is mode 0)", prjMode);
+        return;
+    }
+    if (!hvol || !hprj) {
+        set_error(CTLAB_ERR_ARG, "NULL volume or projection pointer");
+        return;
+    }
+    ctlab_plan* P = plan_create_cone(x0, y0, z0, DNU, DNV, xds, yds, zds, imgXCenter, imgYCenter,
+                                     imgZCenter, hangs, hzPos, PN, XN, YN, ZN, dx, dz, mask, gpunum);
+    host_call(P, hvol, hprj, true);
+    free_plan(P);
+}
+
+void DD3Back_gpu(float x0, float y0, float z0, int DNU, int DNV, float* xds, float* yds, float* zds,
+                 float imgXCenter, float imgYCenter, float imgZCenter, float* hangs, float* hzPos, int PN,
+                 int XN, int YN, int ZN, float* hvol, float* hprj, float dx, float dz, byte* mask,
+                 int gpunum, int squared, int prjMode)
+{
+    (void)squared;
+    clear_error();
+    if (prjMode != 0) {
+        set_error(CTLAB_ERR_MODE, "prjMode %d not supported (branchless DD is mode 0)", prjMode);
+        return;
+    }
+    if (!hvol || !hprj) {
+        set_error(CTLAB_ERR_ARG, "NULL volume or projection pointer");
+        return;
+    }
+    ctlab_plan* P = plan_create_cone(x0, y0, z0, DNU, DNV, xds, yds, zds, imgXCenter, imgYCenter,
+                                     imgZCenter, hangs, hzPos, PN, XN, YN, ZN, dx, dz, mask, gpunum);
+    host_call(P, hvol, hprj, false);
+    free_plan(P);
+}
+
+void DD2_multiGPU(float* hvol, float* hprj, const int method, const float x0, const float y0, float* xds,
+                  float* yds, const int DNU, const int SLN, const float imgXCenter, const float imgYCenter,
+                  const int XN, const int YN, const float dx, float* hangs, int PN, byte* mask,
+                  int* startIdx, const int gpuNum)
+{
+    clear_error();
+    if (method != 0 && method != 2) {
+        set_error(CTLAB_ERR_MODE, "method %d not supported (0: branchless proj, 2: branchless back)", method);
+        return;
+    }
+    if (!hvol || !hprj) {
+        set_error(CTLAB_ERR_ARG, "NULL volume or projection pointer");
+        return;
+    }
+    if (gpuNum < 1) {
+        set_error(CTLAB_ERR_ARG, "gpuNum must be >= 1");
+        return;
+    }
+    if (startIdx) {
+        if (startIdx[0] != 0) {
+            set_error(CTLAB_ERR_ARG, "startIdx[0] must be 0");
+            return;
+        }
+        for (int i = 1; i < gpuNum; ++i)
+            if (startIdx[i] < startIdx[i - 1] || startIdx[i] > SLN) {
+                set_error(CTLAB_ERR_ARG, "startIdx must be non-decreasing within [0, SLN]");
+                return;
+            }
+    }
+    int dev = 0;
+    hipGetDevice(&dev);
+    ctlab_plan* P = plan_create_fan(x0, y0, xds, yds, DNU, SLN, imgXCenter, imgYCenter, XN, YN, dx, hangs,
+                                    PN, mask, dev);
+    host_call(P, hvol, hprj, method == 0);
+    free_plan(P);
+}
+
+}  // extern "C"

@@ -1,0 +1,91 @@
+"""C-ABI checks that need no GPU: the library loads, exports every function
+declared in include/*.h, and rejects bad arguments before touching the device
+(the reference would exit(1) under DEBUG or ignore the error)."""
+import ctypes as C
+import glob
+import os
+import re
+
+import numpy as np
+import pytest
+
+from conftest import ROOT
+from ctlab_amd import _lib, dd
+from ctlab_amd import geometry as G
+
+
+def _declared_functions():
+    names = set()
+    for h in glob.glob(os.path.join(ROOT, "include", "*.h")):
+        txt = open(h).read()
+        txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
+        txt = re.sub(r"//[^\n]*", "", txt)
+        for m in re.finditer(r"\b([A-Za-z_][A-Za-z0-9_]*)\s*\(", txt):
+            nm = m.group(1)
+            if nm in ("if", "sizeof", "defined"):
+                continue
+            pre = txt[:m.start()].rstrip()
+            if pre.endswith("(") or pre.endswith(","):
+                continue
+            names.add(nm)
+    return names
+
+
+def test_library_exports_every_declared_symbol():
+    L = _lib.lib()
+    names = _declared_functions()
+    assert {"DD3Proj_gpu", "DD3Back_gpu", "DD2_multiGPU", "ctlab_last_error"} <= names
+    missing = [n for n in sorted(names) if not hasattr(L, n)]
+    assert not missing, missing
+
+
+def _tiny():
+    return G.circular_cone(XN=8, YN=8, ZN=4, dx=1.0, dz=1.0, PN=4, DNU=12, DNV=4, col_size=1.0,
+                           row_size=1.0, sid=40.0, sdd=70.0)
+
+
+def test_unsupported_mode_sets_error_and_leaves_output():
+    L = _lib.lib()
+    g = _tiny()
+    vol = np.ones(g.nvox(), np.float32)
+    prj = np.full(g.nprj(), 7.0, np.float32)
+    L.DD3Proj_gpu(g.x0, g.y0, g.z0, g.DNU, g.DNV, _lib.fptr(g.xds), _lib.fptr(g.yds), _lib.fptr(g.zds),
+                  0.0, 0.0, 0.0, _lib.fptr(g.angs), _lib.fptr(g.zpos), g.PN, g.XN, g.YN, g.ZN,
+                  _lib.fptr(vol), _lib.fptr(prj), g.dx, g.dz, None, 0, 4)
+    code, msg = _lib.last_error()
+    assert code == 2 and "prjMode" in msg
+    assert np.all(prj == 7.0)
+    L.DD3Back_gpu(g.x0, g.y0, g.z0, g.DNU, g.DNV, _lib.fptr(g.xds), _lib.fptr(g.yds), _lib.fptr(g.zds),
+                  0.0, 0.0, 0.0, _lib.fptr(g.angs), _lib.fptr(g.zpos), g.PN, g.XN, g.YN, g.ZN,
+                  _lib.fptr(vol), _lib.fptr(prj), g.dx, g.dz, None, 0, 0, 3)
+    assert _lib.last_error()[0] == 2
+    assert np.all(vol == 1.0)
+    with pytest.raises(_lib.CtlabError):
+        dd.DD2_multiGPU(G.fan_multislice(8, 8, 1, 1.0, 4, 12, 1.0), np.ones(64), method=1)
+
+
+def test_invalid_sizes_and_null_pointers():
+    L = _lib.lib()
+    g = _tiny()
+    vol = np.ones(g.nvox(), np.float32)
+    prj = np.zeros(g.nprj(), np.float32)
+    L.DD3Proj_gpu(g.x0, g.y0, g.z0, g.DNU, g.DNV, _lib.fptr(g.xds), _lib.fptr(g.yds), _lib.fptr(g.zds),
+                  0.0, 0.0, 0.0, _lib.fptr(g.angs), _lib.fptr(g.zpos), 0, g.XN, g.YN, g.ZN,
+                  _lib.fptr(vol), _lib.fptr(prj), g.dx, g.dz, None, 0, 0)
+    assert _lib.last_error()[0] == 1
+    L.DD3Proj_gpu(g.x0, g.y0, g.z0, g.DNU, g.DNV, None, _lib.fptr(g.yds), _lib.fptr(g.zds),
+                  0.0, 0.0, 0.0, _lib.fptr(g.angs), _lib.fptr(g.zpos), g.PN, g.XN, g.YN, g.ZN,
+                  _lib.fptr(vol), _lib.fptr(prj), g.dx, g.dz, None, 0, 0)
+    assert _lib.last_error()[0] == 1
+    L.DD3Proj_gpu(g.x0, g.y0, g.z0, g.DNU, g.DNV, _lib.fptr(g.xds), _lib.fptr(g.yds), _lib.fptr(g.zds),
+                  0.0, 0.0, 0.0, _lib.fptr(g.angs), _lib.fptr(g.zpos), g.PN, g.XN, g.YN, g.ZN,
+                  None, _lib.fptr(prj), g.dx, g.dz, None, 0, 0)
+    assert _lib.last_error()[0] == 1
+    assert np.all(prj == 0)
+
+
+def test_error_strings():
+    L = _lib.lib()
+    assert L.ctlab_error_string(0) == b"success"
+    assert L.ctlab_error_string(2) == b"unsupported projection mode"
+    assert L.ctlab_error_string(99) == b"unknown error"
