@@ -1,6 +1,33 @@
 """ctypes declarations of the library's recon / learned-prox / batch entry points."""
 import ctypes as C
 
+_vp = C.c_void_p
+_fp = C.c_void_p      # device pointers are passed as integers
+_dp = C.POINTER(C.c_double)
+
 
 def declare(L):
-    pass
+    i, f, d, ll = C.c_int, C.c_float, C.c_double, C.c_longlong
+    sig = {
+        "ctlab_recon_create": ([_vp, i], _vp),
+        "ctlab_recon_destroy": ([_vp], None),
+        "ctlab_recon_set_data": ([_vp, _fp, _vp], i),
+        "ctlab_os_sart": ([_vp, _fp, i, f, i, i, _vp], i),
+        "ctlab_fista_tv": ([_vp, _fp, i, f, i, f, i, _dp, _vp], i),
+        "ctlab_cg": ([_vp, _fp, i, i, _dp, _vp], i),
+        "ctlab_recon_data_fidelity": ([_vp, _fp, _vp], d),
+        "ctlab_recon_gradient": ([_vp, _fp, _fp, _vp], d),
+        "ctlab_recon_lipschitz": ([_vp, i, _vp], d),
+        "ctlab_sart_prj_weight": ([_fp, _fp, _fp, ll, _vp], i),
+        "ctlab_sart_bak_weight": ([_fp, _fp, _fp, ll, _vp], i),
+        "ctlab_ossart_update": ([_fp, _fp, _fp, _fp, f, ll, _vp], i),
+        "ctlab_fista_momentum": ([_fp, _fp, f, f, f, f, ll, _vp], i),
+        "ctlab_soft_threshold": ([_fp, f, ll, _vp], i),
+        "ctlab_cg_beta": ([_fp, _fp, _fp, i, ll, _dp, _vp], i),
+        "ctlab_tv_prox": ([_fp, _fp, f, i, i, i, i, _fp, _vp], i),
+    }
+    for name, (args, res) in sig.items():
+        if hasattr(L, name):
+            fn = getattr(L, name)
+            fn.argtypes = args
+            fn.restype = res

@@ -1,0 +1,475 @@
+// Learned proximal CNN kernels (gfx950): 5 layers of 3x3x3 convolution,
+// 1 -> 32 -> 32 -> 32 -> 32 -> 1, ReLU after layers 1-4, residual output
+// x + net(x).  BF16 weights and activations, FP32 accumulation.  No
+// counterpart in CTLAB (the north star adds it).
+//
+// Layouts (voxel v = (y*XN + x)*ZN + z, the volume's (Z,X,Y) order):
+//   channels-last  act_cl[v*32 + c]   (MFMA A-operand: 8 channels of a voxel = 16 B)
+//   channel-major  act_cm[c*nvox + v] (wgrad operands: 8 voxels of a channel = 16 B)
+//   weights        w[t][co][ci], tap t = (dy+1)*9 + (dx+1)*3 + (dz+1)
+// Hidden 32 -> 32 layers are implicit GEMMs on v_mfma_f32_16x16x32_bf16:
+// one MFMA per (16 voxels, 16 output channels, tap) with K = the 32 input
+// channels.  The input gradient runs the same kernel with flipped, transposed
+// weights and the ReLU mask fused into its epilogue; the weight gradient runs
+// MFMAs over K = voxels on the channel-major copies.  The thin first and last
+// layers run on the VALU.
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace ctlab {
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+
+constexpr int CH = 32;
+
+__device__ __forceinline__ int tap_off(int t, int XN, int ZN)
+{
+    const int dy = t / 9 - 1, dx = (t / 3) % 3 - 1, dz = t % 3 - 1;
+    return dz + dx * ZN + dy * XN * ZN;
+}
+
+__device__ __forceinline__ bool tap_ok(int t, int y, int x, int z, int YN, int XN, int ZN)
+{
+    const int dy = t / 9 - 1, dx = (t / 3) % 3 - 1, dz = t % 3 - 1;
+    return (unsigned)(y + dy) < (unsigned)YN && (unsigned)(x + dx) < (unsigned)XN &&
+           (unsigned)(z + dz) < (unsigned)ZN;
+}
+
+__device__ __forceinline__ float bf(const __bf16 v) { return (float)v; }
+
+// ---------------------------------------------------------------------------
+// 32 -> 32 convolution, MFMA.  Block = 4 wavefronts x 64 voxels.
+//   out = act(conv(in; w) + bias) ; act: relu (relu=1) or identity
+//   mask (optional, channels-last): out *= (mask > 0)   [input-gradient pass]
+// Writes channels-last and (optionally) channel-major copies.
+// ---------------------------------------------------------------------------
+constexpr int kConvMB = 4;           // 16-voxel blocks per wavefront
+
+__global__ void __launch_bounds__(256)
+k_conv32(const __bf16* __restrict__ in, const __bf16* __restrict__ w, const float* __restrict__ bias,
+         const __bf16* __restrict__ mask, __bf16* __restrict__ out_cl, __bf16* __restrict__ out_cm,
+         int XN, int YN, int ZN, int relu)
+{
+    __shared__ bf16x8 sW[27 * CH * CH / 8];        // 55,296 B
+    const long nvox = (long)XN * YN * ZN;
+    for (int i = threadIdx.x; i < 27 * CH * CH / 8; i += 256) sW[i] = ((const bf16x8*)w)[i];
+    __syncthreads();
+
+    const int lane = threadIdx.x & 63;
+    const int wv = threadIdx.x >> 6;
+    const int m = lane & 15, kq = lane >> 4;
+    const long vbase = ((long)blockIdx.x * 4 + wv) * (16 * kConvMB);
+
+    int vy[kConvMB], vx[kConvMB], vz[kConvMB];
+    long vv[kConvMB];
+#pragma unroll
+    for (int b = 0; b < kConvMB; ++b) {
+        const long v = vbase + b * 16 + m;
+        vv[b] = v;
+        const long vc = v < nvox ? v : 0;
+        vz[b] = (int)(vc % ZN);
+        const long xy = vc / ZN;
+        vx[b] = (int)(xy % XN);
+        vy[b] = (int)(xy / XN);
+    }
+    f32x4 acc[kConvMB][2];
+#pragma unroll
+    for (int b = 0; b < kConvMB; ++b) acc[b][0] = acc[b][1] = (f32x4){0.f, 0.f, 0.f, 0.f};
+
+    for (int t = 0; t < 27; ++t) {
+        const int off = tap_off(t, XN, ZN);
+        const bf16x8 b0 = sW[((t * CH) + m) * (CH / 8) + kq];
+        const bf16x8 b1 = sW[((t * CH) + 16 + m) * (CH / 8) + kq];
+#pragma unroll
+        for (int b = 0; b < kConvMB; ++b) {
+            bf16x8 a = (bf16x8){0, 0, 0, 0, 0, 0, 0, 0};
+            if (vv[b] < nvox && tap_ok(t, vy[b], vx[b], vz[b], YN, XN, ZN))
+                a = *(const bf16x8*)(in + (vv[b] + off) * CH + 8 * kq);
+            acc[b][0] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b0, acc[b][0], 0, 0, 0);
+            acc[b][1] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b1, acc[b][1], 0, 0, 0);
+        }
+    }
+
+    // D fragment: lane holds rows 4*kq + i (voxels), column m (output channel)
+#pragma unroll
+    for (int b = 0; b < kConvMB; ++b) {
+#pragma unroll
+        for (int nb = 0; nb < 2; ++nb) {
+            const int co = nb * 16 + m;
+            const float bco = bias ? bias[co] : 0.0f;
+            __bf16 o[4];
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                const long v = vbase + b * 16 + 4 * kq + i;
+                float r = acc[b][nb][i] + bco;
+                if (relu) r = fmaxf(r, 0.0f);
+                if (mask && v < nvox && !(bf(mask[v * CH + co]) > 0.0f)) r = 0.0f;
+                o[i] = (__bf16)r;
+                if (v < nvox) out_cl[v * CH + co] = o[i];
+            }
+            if (out_cm) {
+                const long v0 = vbase + b * 16 + 4 * kq;
+                if (v0 + 3 < nvox) {
+                    typedef __bf16 bf16x4 __attribute__((ext_vector_type(4)));
+                    *(bf16x4*)(out_cm + (long)co * nvox + v0) = (bf16x4){o[0], o[1], o[2], o[3]};
+                } else {
+                    for (int i = 0; i < 4; ++i)
+                        if (v0 + i < nvox) out_cm[(long)co * nvox + v0 + i] = o[i];
+                }
+            }
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// first layer 1 -> 32 (VALU): out = relu(conv(x; w1) + b1); w1 fp32 [27][32]
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(256)
+k_conv_first(const float* __restrict__ x, const float* __restrict__ w1, const float* __restrict__ b1,
+             __bf16* __restrict__ out_cl, __bf16* __restrict__ out_cm, int XN, int YN, int ZN)
+{
+    __shared__ float sW[27 * CH + CH];
+    for (int i = threadIdx.x; i < 27 * CH; i += 256) sW[i] = w1[i];
+    for (int i = threadIdx.x; i < CH; i += 256) sW[27 * CH + i] = b1[i];
+    __syncthreads();
+    const long nvox = (long)XN * YN * ZN;
+    const long v = blockIdx.x * 256L + threadIdx.x;
+    if (v >= nvox) return;
+    const int z = (int)(v % ZN);
+    const long xy = v / ZN;
+    const int xx = (int)(xy % XN), y = (int)(xy / XN);
+    float acc[CH];
+#pragma unroll
+    for (int c = 0; c < CH; ++c) acc[c] = sW[27 * CH + c];
+    for (int t = 0; t < 27; ++t) {
+        if (!tap_ok(t, y, xx, z, YN, XN, ZN)) continue;
+        const float a = x[v + tap_off(t, XN, ZN)];
+#pragma unroll
+        for (int c = 0; c < CH; ++c) acc[c] += sW[t * CH + c] * a;
+    }
+    __bf16 o[CH];
+#pragma unroll
+    for (int c = 0; c < CH; ++c) {
+        o[c] = (__bf16)fmaxf(acc[c], 0.0f);
+        if (out_cm) out_cm[(long)c * nvox + v] = o[c];
+    }
+#pragma unroll
+    for (int q = 0; q < CH / 8; ++q)
+        *(bf16x8*)(out_cl + v * CH + 8 * q) = (bf16x8){o[8 * q], o[8 * q + 1], o[8 * q + 2], o[8 * q + 3],
+                                                        o[8 * q + 4], o[8 * q + 5], o[8 * q + 6], o[8 * q + 7]};
+}
+
+// ---------------------------------------------------------------------------
+// last layer 32 -> 1 (VALU) + residual: out = x + conv(a4; w5) + b5
+// (w5 fp32 [27][32]); with target != null also g5 = (out - target) * gscale
+// and the squared residual per block (FP64) for the loss.
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(256)
+k_conv_last(const __bf16* __restrict__ in, const float* __restrict__ w5, const float* __restrict__ b5,
+            const float* __restrict__ x, float* __restrict__ out, const float* __restrict__ target,
+            float* __restrict__ g5, float gscale, double* __restrict__ part, int XN, int YN, int ZN)
+{
+    __shared__ float sW[27 * CH];
+    __shared__ double sred[4];
+    for (int i = threadIdx.x; i < 27 * CH; i += 256) sW[i] = w5[i];
+    __syncthreads();
+    const long nvox = (long)XN * YN * ZN;
+    const long v = blockIdx.x * 256L + threadIdx.x;
+    double r2 = 0.0;
+    if (v < nvox) {
+        const int z = (int)(v % ZN);
+        const long xy = v / ZN;
+        const int xx = (int)(xy % XN), y = (int)(xy / XN);
+        float acc = b5[0];
+        for (int t = 0; t < 27; ++t) {
+            if (!tap_ok(t, y, xx, z, YN, XN, ZN)) continue;
+            const bf16x8* a = (const bf16x8*)(in + (v + tap_off(t, XN, ZN)) * CH);
+#pragma unroll
+            for (int q = 0; q < CH / 8; ++q) {
+                const bf16x8 av = a[q];
+#pragma unroll
+                for (int e = 0; e < 8; ++e) acc += sW[t * CH + 8 * q + e] * (float)av[e];
+            }
+        }
+        const float o = x[v] + acc;
+        out[v] = o;
+        if (target) {
+            const float r = o - target[v];
+            g5[v] = r * gscale;
+            r2 = (double)r * r;
+        }
+    }
+    if (part) {
+        for (int s = 32; s > 0; s >>= 1) r2 += __shfl_xor(r2, s, 64);
+        if ((threadIdx.x & 63) == 0) sred[threadIdx.x >> 6] = r2;
+        __syncthreads();
+        if (threadIdx.x == 0) part[blockIdx.x] = sred[0] + sred[1] + sred[2] + sred[3];
+    }
+}
+
+// input gradient of the last layer, masked by layer-4 activations:
+// g4[v][ci] = (a4[v][ci] > 0) * sum_t w5[t][ci] * g5[v - off(t)]
+__global__ void __launch_bounds__(256)
+k_dgrad_last(const float* __restrict__ g5, const float* __restrict__ w5, const __bf16* __restrict__ a4,
+             __bf16* __restrict__ g_cl, __bf16* __restrict__ g_cm, int XN, int YN, int ZN)
+{
+    __shared__ float sW[27 * CH];
+    for (int i = threadIdx.x; i < 27 * CH; i += 256) sW[i] = w5[i];
+    __syncthreads();
+    const long nvox = (long)XN * YN * ZN;
+    const long v = blockIdx.x * 256L + threadIdx.x;
+    if (v >= nvox) return;
+    const int z = (int)(v % ZN);
+    const long xy = v / ZN;
+    const int xx = (int)(xy % XN), y = (int)(xy / XN);
+    float acc[CH];
+#pragma unroll
+    for (int c = 0; c < CH; ++c) acc[c] = 0.0f;
+    for (int t = 0; t < 27; ++t) {
+        // out[v'] used in[v] through tap t when v' = v - off(t)
+        const int tt = 26 - t;   // tap with the opposite offset
+        if (!tap_ok(tt, y, xx, z, YN, XN, ZN)) continue;
+        const float gv = g5[v + tap_off(tt, XN, ZN)];
+#pragma unroll
+        for (int c = 0; c < CH; ++c) acc[c] += sW[t * CH + c] * gv;
+    }
+    __bf16 o[CH];
+    const bf16x8* am = (const bf16x8*)(a4 + v * CH);
+#pragma unroll
+    for (int q = 0; q < CH / 8; ++q) {
+        const bf16x8 mv = am[q];
+#pragma unroll
+        for (int e = 0; e < 8; ++e) {
+            const int c = 8 * q + e;
+            o[c] = (__bf16)((float)mv[e] > 0.0f ? acc[c] : 0.0f);
+            g_cm[(long)c * nvox + v] = o[c];
+        }
+    }
+#pragma unroll
+    for (int q = 0; q < CH / 8; ++q)
+        *(bf16x8*)(g_cl + v * CH + 8 * q) = (bf16x8){o[8 * q], o[8 * q + 1], o[8 * q + 2], o[8 * q + 3],
+                                                     o[8 * q + 4], o[8 * q + 5], o[8 * q + 6], o[8 * q + 7]};
+}
+
+// ---------------------------------------------------------------------------
+// weight gradient of a 32 -> 32 layer on MFMA, K = voxels:
+//   dW[t][co][ci] = sum_v g[v][co] * a[v + off(t)][ci]
+// from channel-major g_cm, a_cm.  One wavefront = one (dy, dx) pair (3 taps,
+// dz = -1, 0, +1) over a chunk of voxels; needs ZN % 8 == 0 so that a run of
+// 8 voxels stays in one z-column.  The dz = +-1 runs are the aligned run
+// shifted by one element, assembled with alignbyte from a 16-byte load and one
+// neighbouring dword.  Partial sums per chunk: part[chunk][27][32][32].
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ bf16x8 as_bf(u32x4 u)
+{
+    return __builtin_bit_cast(bf16x8, u);
+}
+
+__global__ void __launch_bounds__(64)
+k_wgrad32(const __bf16* __restrict__ g_cm, const __bf16* __restrict__ a_cm, float* __restrict__ part,
+          int XN, int YN, int ZN, long chunk)
+{
+    const long nvox = (long)XN * YN * ZN;
+    const int lane = threadIdx.x;
+    const int m = lane & 15, kq = lane >> 4;
+    const int pair = blockIdx.y;                   // 0..8 = (dy+1)*3 + (dx+1)
+    const int dy = pair / 3 - 1, dx = pair % 3 - 1;
+    const long c0 = blockIdx.x * chunk;
+    const long c1 = min(c0 + chunk, nvox);
+    f32x4 acc[3][2][2];
+#pragma unroll
+    for (int d = 0; d < 3; ++d)
+        for (int i = 0; i < 2; ++i)
+            for (int j = 0; j < 2; ++j) acc[d][i][j] = (f32x4){0.f, 0.f, 0.f, 0.f};
+    const long shift = (long)dx * ZN + (long)dy * XN * ZN;
+    const uint32_t* a32 = (const uint32_t*)a_cm;
+    for (long v0 = c0; v0 < c1; v0 += 32) {
+        const long vr = v0 + 8 * kq;                 // this lane's run of 8 voxels
+        const int z0 = (int)(vr % ZN);
+        const long xy = vr / ZN;
+        const int xx = (int)(xy % XN), y = (int)(xy / XN);
+        const bool ok = vr < nvox && (unsigned)(y + dy) < (unsigned)YN && (unsigned)(xx + dx) < (unsigned)XN;
+        bf16x8 A0 = *(const bf16x8*)(g_cm + (long)m * nvox + (vr < nvox ? vr : 0));
+        bf16x8 A1 = *(const bf16x8*)(g_cm + (long)(16 + m) * nvox + (vr < nvox ? vr : 0));
+        if (vr >= nvox) A0 = A1 = (bf16x8){0, 0, 0, 0, 0, 0, 0, 0};
+#pragma unroll
+        for (int nb = 0; nb < 2; ++nb) {
+            const int ci = nb * 16 + m;
+            u32x4 blk = (u32x4){0u, 0u, 0u, 0u};
+            uint32_t prev = 0u, next = 0u;
+            if (ok) {
+                const long e = (long)ci * nvox + vr + shift;     // 8-aligned element index
+                blk = *(const u32x4*)(a_cm + e);
+                const long ep = e - 2 >= 0 ? e - 2 : 0;
+                const long en = e + 8 <= (long)CH * nvox - 2 ? e + 8 : e;
+                prev = a32[ep >> 1];
+                next = a32[en >> 1];
+            }
+            // dz = 0
+            const bf16x8 B0 = as_bf(blk);
+            // dz = -1: elements e-1 .. e+6 (element 0 invalid at z0 == 0)
+            u32x4 lo;
+            lo[0] = __builtin_amdgcn_alignbyte(blk[0], prev, 2);
+            lo[1] = __builtin_amdgcn_alignbyte(blk[1], blk[0], 2);
+            lo[2] = __builtin_amdgcn_alignbyte(blk[2], blk[1], 2);
+            lo[3] = __builtin_amdgcn_alignbyte(blk[3], blk[2], 2);
+            if (z0 == 0) lo[0] &= 0xffff0000u;
+            // dz = +1: elements e+1 .. e+8 (element 7 invalid at z0 + 8 == ZN)
+            u32x4 hi;
+            hi[0] = __builtin_amdgcn_alignbyte(blk[1], blk[0], 2);
+            hi[1] = __builtin_amdgcn_alignbyte(blk[2], blk[1], 2);
+            hi[2] = __builtin_amdgcn_alignbyte(blk[3], blk[2], 2);
+            hi[3] = __builtin_amdgcn_alignbyte(next, blk[3], 2);
+            if (z0 + 8 == ZN) hi[3] &= 0x0000ffffu;
+            const bf16x8 Bm = as_bf(lo), Bp = as_bf(hi);
+            acc[0][0][nb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A0, Bm, acc[0][0][nb], 0, 0, 0);
+            acc[0][1][nb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A1, Bm, acc[0][1][nb], 0, 0, 0);
+            acc[1][0][nb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A0, B0, acc[1][0][nb], 0, 0, 0);
+            acc[1][1][nb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A1, B0, acc[1][1][nb], 0, 0, 0);
+            acc[2][0][nb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A0, Bp, acc[2][0][nb], 0, 0, 0);
+            acc[2][1][nb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A1, Bp, acc[2][1][nb], 0, 0, 0);
+        }
+    }
+    // D: rows co = mb*16 + 4*kq + i, column ci = nb*16 + m
+    float* P = part + (long)blockIdx.x * 27 * CH * CH;
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        const int t = pair * 3 + d;
+#pragma unroll
+        for (int mb = 0; mb < 2; ++mb)
+#pragma unroll
+            for (int nb = 0; nb < 2; ++nb)
+#pragma unroll
+                for (int i = 0; i < 4; ++i)
+                    P[((long)t * CH + mb * 16 + 4 * kq + i) * CH + nb * 16 + m] = acc[d][mb][nb][i];
+    }
+}
+
+// weight gradient of the first layer: dW1[t][co] = sum_v g1[v][co] * x[v + off(t)]
+// one block per chunk; thread (t, co) pairs; partials part[chunk][27*32]
+__global__ void __launch_bounds__(256)
+k_wgrad_first(const __bf16* __restrict__ g_cm, const float* __restrict__ x, float* __restrict__ part,
+              int XN, int YN, int ZN, long chunk)
+{
+    const long nvox = (long)XN * YN * ZN;
+    const long c0 = blockIdx.x * chunk, c1 = min(c0 + chunk, nvox);
+    float acc[4] = {0.f, 0.f, 0.f, 0.f};
+    for (long v = c0; v < c1; ++v) {
+        const int z = (int)(v % ZN);
+        const long xy = v / ZN;
+        const int xx = (int)(xy % XN), y = (int)(xy / XN);
+        for (int k = 0; k < 4; ++k) {
+            const int idx = threadIdx.x + 256 * k;   // t*32 + co
+            if (idx >= 27 * CH) break;
+            const int t = idx / CH, co = idx % CH;
+            if (tap_ok(t, y, xx, z, YN, XN, ZN)) acc[k] += bf(g_cm[(long)co * nvox + v]) * x[v + tap_off(t, XN, ZN)];
+        }
+    }
+    for (int k = 0; k < 4; ++k) {
+        const int idx = threadIdx.x + 256 * k;
+        if (idx < 27 * CH) part[(long)blockIdx.x * 27 * CH + idx] = acc[k];
+    }
+}
+
+// weight gradient of the last layer: dW5[t][ci] = sum_v g5[v] * a4[v + off(t)][ci]
+__global__ void __launch_bounds__(256)
+k_wgrad_last(const float* __restrict__ g5, const __bf16* __restrict__ a_cl, float* __restrict__ part,
+             int XN, int YN, int ZN, long chunk)
+{
+    const long nvox = (long)XN * YN * ZN;
+    const long c0 = blockIdx.x * chunk, c1 = min(c0 + chunk, nvox);
+    float acc[4] = {0.f, 0.f, 0.f, 0.f};
+    for (long v = c0; v < c1; ++v) {
+        const int z = (int)(v % ZN);
+        const long xy = v / ZN;
+        const int xx = (int)(xy % XN), y = (int)(xy / XN);
+        const float gv = g5[v];
+        for (int k = 0; k < 4; ++k) {
+            const int idx = threadIdx.x + 256 * k;   // t*32 + ci
+            if (idx >= 27 * CH) break;
+            const int t = idx / CH, ci = idx % CH;
+            if (tap_ok(t, y, xx, z, YN, XN, ZN)) acc[k] += gv * bf(a_cl[(v + tap_off(t, XN, ZN)) * CH + ci]);
+        }
+    }
+    for (int k = 0; k < 4; ++k) {
+        const int idx = threadIdx.x + 256 * k;
+        if (idx < 27 * CH) part[(long)blockIdx.x * 27 * CH + idx] = acc[k];
+    }
+}
+
+// bias gradients: db[c] = sum_v g_cm[c][v]  (one block per channel)
+__global__ void __launch_bounds__(256)
+k_bias_grad_cm(const __bf16* __restrict__ g_cm, long nvox, float* __restrict__ db)
+{
+    __shared__ float sh[4];
+    float s = 0.0f;
+    for (long v = threadIdx.x; v < nvox; v += 256) s += bf(g_cm[(long)blockIdx.x * nvox + v]);
+    for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
+    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) db[blockIdx.x] = sh[0] + sh[1] + sh[2] + sh[3];
+}
+
+__global__ void __launch_bounds__(256)
+k_sum_f32(const float* __restrict__ x, long n, float* __restrict__ out)
+{
+    __shared__ float sh[4];
+    float s = 0.0f;
+    for (long v = threadIdx.x; v < n; v += 256) s += x[v];
+    for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
+    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) out[0] = sh[0] + sh[1] + sh[2] + sh[3];
+}
+
+// out[i] = sum_c part[c][i]  (fixed order over chunks)
+__global__ void k_reduce_chunks(const float* __restrict__ part, int nchunks, int n, float* __restrict__ out)
+{
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    float s = 0.0f;
+    for (int c = 0; c < nchunks; ++c) s += part[(long)c * n + i];
+    out[i] = s;
+}
+
+// flipped, transposed weights for the input gradient: wt[t][ci][co] = w[26-t][co][ci]
+__global__ void k_flip_transpose(const __bf16* __restrict__ w, __bf16* __restrict__ wt)
+{
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= 27 * CH * CH) return;
+    const int t = i / (CH * CH), ci = (i / CH) % CH, co = i % CH;
+    wt[i] = w[((26 - t) * CH + co) * CH + ci];
+}
+
+// Adam on FP32 master parameters; also refreshes the BF16 copies the kernels use
+// (flat parameter vector, per-layer BF16 images written by index maps)
+__global__ void k_adam(float* __restrict__ p, const float* __restrict__ g, float* __restrict__ m1,
+                       float* __restrict__ m2, long n, float lr, float b1, float b2, float eps, float c1,
+                       float c2)
+{
+    const long i = blockIdx.x * 256L + threadIdx.x;
+    if (i >= n) return;
+    const float gi = g[i];
+    const float a = b1 * m1[i] + (1.0f - b1) * gi;
+    const float b = b2 * m2[i] + (1.0f - b2) * gi * gi;
+    m1[i] = a;
+    m2[i] = b;
+    p[i] -= lr * (a / c1) / (sqrtf(b / c2) + eps);
+}
+
+// bf16 image of an fp32 parameter block (rounded to nearest even), and an fp32
+// copy holding the same rounded values for the VALU layers
+__global__ void k_to_bf16(const float* __restrict__ src, __bf16* __restrict__ dst_bf, float* __restrict__ dst_f,
+                          long n)
+{
+    const long i = blockIdx.x * 256L + threadIdx.x;
+    if (i >= n) return;
+    const __bf16 b = (__bf16)src[i];
+    if (dst_bf) dst_bf[i] = b;
+    if (dst_f) dst_f[i] = (float)b;
+}
+
+}  // namespace ctlab

@@ -37,6 +37,8 @@ struct DevGeom {
     const double* tau;        // PN*DNU: 1 / (cell-centre slab coord - source slab coord)
     const double* colfac;     // PN*DNU: per-column output factor
     const double* exy2;       // PN*DNU: squared in-plane source->cell-centre distance
+    const double* invdk;      // PN*DNU: 1/|kap[u+1]-kap[u]|
+    const double* esdz;       // PN*DNU: (cell-centre slab coord - source slab coord) / dzh
     const double* zhat;       // DNV+1: row boundaries, (z - z0)/dz (cone) or v (fan)
     const double* ez;         // DNV: row centre z - z0 (0 for fan beam)
     const double* rowfac;     // DNV: 1/(zhat[v+1]-zhat[v]) (cone) or 1 (fan)

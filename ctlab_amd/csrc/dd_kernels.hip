@@ -28,29 +28,25 @@
 
 namespace ctlab {
 
-template <typename T>
-__device__ __forceinline__ T shfl64(T v, int src)
+// Intra-wavefront hand-off through LDS: every lane writes, then every lane
+// reads other lanes' values.  LDS executes one wavefront's instructions in
+// order; the fences keep the compiler from moving the accesses.
+__device__ __forceinline__ void wave_lds_sync()
 {
-    return __shfl(v, src, 64);
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-// G[i] with G[0] = 0 and G[i] = H of lane i-1 (H = inclusive prefix, lane k -> k+1)
+// piecewise-linear read of a prefix table G[0..n] (G[0] = 0) at b in [0, n]
 template <typename T>
-__device__ __forceinline__ T prefix_at(T H, int i)
-{
-    const T v = shfl64(H, (i - 1) & 63);
-    return i == 0 ? T(0) : v;
-}
-
-// piecewise-linear read of the prefix G at fractional coordinate b in [0, n]
-template <typename T>
-__device__ __forceinline__ T interp_prefix(T H, double b, int n)
+__device__ __forceinline__ T lds_interp(const T* G, double b, int n)
 {
     b = fmin(fmax(b, 0.0), (double)n);
     const int i = min((int)b, n - 1);
     const T fr = (T)(b - (double)i);
-    const T g0 = prefix_at(H, i);
-    const T g1 = prefix_at(H, i + 1);
+    const T g0 = G[i];
+    const T g1 = G[i + 1];
     return g0 + fr * (g1 - g0);
 }
 
@@ -75,21 +71,26 @@ zprefix_kernel(const float* __restrict__ vol, T* __restrict__ cz,
 }
 
 // ---------------------------------------------------------------------------
-// Projector walk.  One wavefront = (view p, tile of NU detector columns),
-// looping over all slabs; lane = z for the SAT rows, lane = v for outputs.
+// Projector walk.  Block = 4 wavefronts = 4 consecutive views of one tile of
+// NU detector columns (neighbouring views read nearly the same slab rows);
+// each wavefront loops over all slabs of its view.  Lane = z for the SAT rows,
+// lane = v for outputs.
 // ---------------------------------------------------------------------------
 template <typename T, int NU>
 __global__ void __launch_bounds__(256)
-dd_proj_walk(DevGeom g, const T* __restrict__ cz, float* __restrict__ prj, int ntiles)
+dd_proj_walk(DevGeom g, const T* __restrict__ cz, float* __restrict__ prj)
 {
+    __shared__ T sG[4][72];
     const int lane = threadIdx.x & 63;
-    const int tile = __builtin_amdgcn_readfirstlane(blockIdx.x * 4 + (threadIdx.x >> 6));
-    const int p = blockIdx.y;
-    if (tile >= ntiles) return;
+    const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int p = blockIdx.y * 4 + wv;
+    if (p >= g.PN) return;
+    T* G = sG[wv];
+    if (lane == 0) G[0] = T(0);
     const ViewParams vp = g.vp[p];
     const double* __restrict__ kap = g.kap + (long)p * (g.DNU + 1);
     const double* __restrict__ tau = g.tau + (long)p * g.DNU;
-    const int u0 = tile * NU;
+    const int u0 = blockIdx.x * NU;
     const int nu = min(NU, g.DNU - u0);
     const int ZN = g.ZN, NR = vp.NR, DNV = g.DNV;
     const int zl = min(lane, ZN - 1);
@@ -98,6 +99,7 @@ dd_proj_walk(DevGeom g, const T* __restrict__ cz, float* __restrict__ prj, int n
     const double inv_dx = 1.0 / g.dx;
     const long slab_off = (long)vp.slab_stride * ZN;
     const long row_off = (long)vp.row_stride * ZN;
+    const long last = (long)(NR - 1) * row_off;
 
     T acc[NU];
 #pragma unroll
@@ -113,28 +115,38 @@ dd_proj_walk(DevGeom g, const T* __restrict__ cz, float* __restrict__ prj, int n
         const T* __restrict__ base = cz + j * slab_off + zl;
         double ap = fmin(fmax(afirst, 0.0), (double)NR);
         int m = min((int)ap, NR - 1);
+        double mf = (double)m;
+        long off = m * row_off;
+        // rows m, m+1, m+2 in flight (indices clamped to the last row)
+        T c0 = base[off];
+        T c1 = base[min(off + row_off, last)];
+        T c2 = base[min(off + 2 * row_off, last)];
         T S = T(0);
-        T c = base[m * row_off];
-        T Eprev = S + (T)(ap - m) * c;
-#pragma unroll
-        for (int q = 1; q <= NU; ++q) {
-            if (q > nu) break;
+        T Eprev = (T)(ap - mf) * c0;
+        for (int q = 1; q <= nu; ++q) {
             double aq = vp.rbase + rs * kap[vp.asc ? u0 + q : u0 + nu - q];
             aq = fmin(fmax(aq, 0.0), (double)NR);
-            while (m < NR - 1 && aq >= (double)(m + 1)) {
-                S += c;
+            while (m < NR - 1 && aq >= mf + 1.0) {
+                S += c0;
                 ++m;
-                c = base[m * row_off];
+                mf += 1.0;
+                off += row_off;
+                c0 = c1;
+                c1 = c2;
+                c2 = base[min(off + 2 * row_off, last)];
             }
-            const T E = S + (T)(aq - m) * c;
+            const T E = S + (T)(aq - mf) * c0;
             const T H = E - Eprev;
             Eprev = E;
             if (aq > ap) {
                 const int u = vp.asc ? u0 + q - 1 : u0 + nu - q;
                 const double t = g.cone ? sj * tau[u] : 1.0;
-                const T Dhi = interp_prefix(H, vp.beta0 + t * zhi, ZN);
-                const T Dlo = interp_prefix(H, vp.beta0 + t * zlo, ZN);
+                G[lane + 1] = H;
+                wave_lds_sync();
+                const T Dhi = lds_interp(G, vp.beta0 + t * zhi, ZN);
+                const T Dlo = lds_interp(G, vp.beta0 + t * zlo, ZN);
                 acc[q - 1] += (Dhi - Dlo) * qj;
+                wave_lds_sync();
             }
             ap = aq;
         }
@@ -178,47 +190,53 @@ prjprefix_kernel(DevGeom g, const float* __restrict__ prj, T* __restrict__ P)
 }
 
 // Adjoint z-part for detector column u: sum_v y*L * (fraction of cell v covered
-// by voxel z-interval [lane, lane+1] mapped onto the detector rows).
+// by the voxel z-interval [lane, lane+1] mapped onto the detector rows), read
+// from the column's v-prefix Pu (lane v holds prefix v+1) through LDS.
 template <typename T>
-__device__ __forceinline__ T column_delta(const DevGeom& g, const T* __restrict__ P, int p, int u,
-                                          double sj, double beta0, int lane)
+__device__ __forceinline__ T column_delta(const DevGeom& g, T* G, T Pu, int p, int u, double inv_sj,
+                                          double beta0, int lane)
 {
     const int DNV = g.DNV;
-    const T Pu = P[((long)p * g.DNU + u) * DNV + min(lane, DNV - 1)];
+    G[lane + 1] = Pu;
+    wave_lds_sync();
     double nlo, nhi;
     T sgn = T(1);
     if (g.cone) {
-        const double t = sj * g.tau[(long)p * g.DNU + u];
-        const double inv = 1.0 / (t * g.dzh);
+        const double inv = inv_sj * g.esdz[(long)p * g.DNU + u];   // 1 / (t * dzh)
         const double off = -beta0 * inv - g.zhat0 / g.dzh;
         nlo = (double)lane * inv + off;
-        nhi = (double)(lane + 1) * inv + off;
+        nhi = nlo + inv;
         if (g.dzh < 0.0) sgn = T(-1);
     } else {
         nlo = (double)lane;
         nhi = (double)(lane + 1);
     }
-    const T Phi = interp_prefix(Pu, nhi, DNV);
-    const T Plo = interp_prefix(Pu, nlo, DNV);
-    return (Phi - Plo) * sgn;
+    const T r = (lds_interp(G, nhi, DNV) - lds_interp(G, nlo, DNV)) * sgn;
+    wave_lds_sync();
+    return r;
 }
 
-// One wavefront = (slab j, tile of NM rows) of one slab orientation, looping
-// over that orientation's views; lane = z.
+// Block = 4 wavefronts = 4 consecutive slabs of one tile of NM rows (their
+// detector-column ranges nearly coincide); each wavefront loops over the views
+// of one slab orientation; lane = z.
 template <typename T, int NM>
 __global__ void __launch_bounds__(256)
 dd_back_walk(DevGeom g, const T* __restrict__ P, float* __restrict__ vol,
              const int* __restrict__ views, int nviews, int yslab, int accumulate,
              int NS, int NR, int ntr)
 {
+    __shared__ T sG[4][72];
     const int lane = threadIdx.x & 63;
-    const int w = __builtin_amdgcn_readfirstlane(blockIdx.x * 4 + (threadIdx.x >> 6));
-    const int j = w / ntr;
-    const int rt = w - j * ntr;
+    const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int rt = blockIdx.x % ntr;
+    const int j = (blockIdx.x / ntr) * 4 + wv;
     if (j >= NS) return;
+    T* G = sG[wv];
+    if (lane == 0) G[0] = T(0);
     const int r0 = rt * NM;
     const int nm = min(NM, NR - r0);
-    const int ZN = g.ZN, DNU = g.DNU;
+    const int ZN = g.ZN, DNU = g.DNU, DNV = g.DNV;
+    const int vl = min(lane, DNV - 1);
     const double inv_dx = 1.0 / g.dx;
 
     T acc[NM];
@@ -229,6 +247,8 @@ dd_back_walk(DevGeom g, const T* __restrict__ P, float* __restrict__ vol,
         const int p = views[iv];
         const ViewParams vp = g.vp[p];
         const double* __restrict__ kap = g.kap + (long)p * (DNU + 1);
+        const double* __restrict__ invdk = g.invdk + (long)p * DNU;
+        const T* __restrict__ Pv = P + (long)p * DNU * DNV + vl;
         const double sj = vp.s0 + j * g.dx;
         const double rs = sj * inv_dx;
         const int asc = vp.asc;
@@ -241,8 +261,15 @@ dd_back_walk(DevGeom g, const T* __restrict__ P, float* __restrict__ vol,
             if (A(mid) > (double)r0) hi = mid; else lo = mid + 1;
         }
         int c = lo - 1;
+        const double inv_sj = 1.0 / sj;
+        const double inv_rs = fabs(g.dx * inv_sj);
+        auto ucol = [&](int cc) { return asc ? cc : DNU - 1 - cc; };
+        // P rows of columns c and c+1 in flight
+        T Pr0 = Pv[(long)ucol(c) * DNV];
+        T Pr1 = Pv[(long)ucol(min(c + 1, DNU - 1)) * DNV];
         double alo = A(c), ahi = A(c + 1);
-        T D = column_delta(g, P, p, asc ? c : DNU - 1 - c, sj, vp.beta0, lane);
+        double iw = inv_rs * invdk[ucol(c)];
+        T D = column_delta(g, G, Pr0, p, ucol(c), inv_sj, vp.beta0, lane);
         T Tsum = T(0), Wprev = T(0);
 #pragma unroll
         for (int i = 0; i <= NM; ++i) {
@@ -253,9 +280,12 @@ dd_back_walk(DevGeom g, const T* __restrict__ P, float* __restrict__ vol,
                 ++c;
                 alo = ahi;
                 ahi = A(c + 1);
-                D = column_delta(g, P, p, asc ? c : DNU - 1 - c, sj, vp.beta0, lane);
+                Pr0 = Pr1;
+                Pr1 = Pv[(long)ucol(min(c + 1, DNU - 1)) * DNV];
+                iw = inv_rs * invdk[ucol(c)];
+                D = column_delta(g, G, Pr0, p, ucol(c), inv_sj, vp.beta0, lane);
             }
-            const double fr = fmin(fmax((mp - alo) / (ahi - alo), 0.0), 1.0);
+            const double fr = fmin(fmax((mp - alo) * iw, 0.0), 1.0);
             const T W = Tsum + (T)fr * D;
             if (i > 0) acc[i - 1] += W - Wprev;
             Wprev = W;
@@ -290,8 +320,8 @@ hipError_t launch_proj(const DevGeom& g, const float* d_vol, void* d_cz, float* 
     hipLaunchKernelGGL(zprefix_kernel<T>, dim3((unsigned)((ncol + 255) / 256)), dim3(256), 0, s,
                        d_vol, cz, g.mask, ncol, g.ZN);
     const int ntiles = (g.DNU + kProjNU - 1) / kProjNU;
-    hipLaunchKernelGGL((dd_proj_walk<T, kProjNU>), dim3((ntiles + 3) / 4, g.PN), dim3(256), 0, s,
-                       g, cz, d_prj, ntiles);
+    hipLaunchKernelGGL((dd_proj_walk<T, kProjNU>), dim3(ntiles, (g.PN + 3) / 4), dim3(256), 0, s,
+                       g, cz, d_prj);
     return hipGetLastError();
 }
 
@@ -306,16 +336,14 @@ hipError_t launch_back(const DevGeom& g, const float* d_prj, void* d_P, float* d
     // pass 1: y-slab views (slab = y, rows along x); overwrites
     {
         const int NS = g.YN, NR = g.XN, ntr = (NR + kBackNM - 1) / kBackNM;
-        const long waves = (long)NS * ntr;
-        hipLaunchKernelGGL((dd_back_walk<T, kBackNM>), dim3((unsigned)((waves + 3) / 4)), dim3(256), 0, s,
-                           g, P, d_vol, d_views_y, ny, 1, 0, NS, NR, ntr);
+        hipLaunchKernelGGL((dd_back_walk<T, kBackNM>), dim3((unsigned)(ntr * ((NS + 3) / 4))), dim3(256), 0,
+                           s, g, P, d_vol, d_views_y, ny, 1, 0, NS, NR, ntr);
     }
     // pass 2: x-slab views (slab = x, rows along y); accumulates
     {
         const int NS = g.XN, NR = g.YN, ntr = (NR + kBackNM - 1) / kBackNM;
-        const long waves = (long)NS * ntr;
-        hipLaunchKernelGGL((dd_back_walk<T, kBackNM>), dim3((unsigned)((waves + 3) / 4)), dim3(256), 0, s,
-                           g, P, d_vol, d_views_x, nx, 0, 1, NS, NR, ntr);
+        hipLaunchKernelGGL((dd_back_walk<T, kBackNM>), dim3((unsigned)(ntr * ((NS + 3) / 4))), dim3(256), 0,
+                           s, g, P, d_vol, d_views_x, nx, 0, 1, NS, NR, ntr);
     }
     return hipGetLastError();
 }

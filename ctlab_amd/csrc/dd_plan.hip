@@ -12,15 +12,11 @@
 
 #include "../../include/ctlab_mi355x.h"
 #include "dd_common.hpp"
+#include "plan.hpp"
 #include "status.hpp"
 
 namespace ctlab {
 
-template <typename T>
-hipError_t launch_proj(const DevGeom& g, const float* d_vol, void* d_cz, float* d_prj, hipStream_t s);
-template <typename T>
-hipError_t launch_back(const DevGeom& g, const float* d_prj, void* d_P, float* d_vol,
-                       const int* d_views_y, int ny, const int* d_views_x, int nx, hipStream_t s);
 
 // ---------------------------------------------------------------------------
 // status (per host thread)
@@ -73,49 +69,7 @@ static double regularize_angle(double a)
     return a;
 }
 
-struct GeomIn {
-    double x0, y0, z0;
-    int DNU, DNV;
-    std::vector<double> xds, yds, zds, angs, zpos;
-    double cx, cy, cz;
-    int PN, XN, YN, ZN;
-    double dx, dz;
-    std::vector<unsigned char> mask;  // empty = none
-    int cone;
-};
-
 }  // namespace ctlab
-
-struct ctlab_plan {
-    int device = 0;
-    ctlab::GeomIn in;
-    std::vector<ctlab::ViewParams> vp;
-    std::vector<double> kap, tau, colfac, exy2, zhat, ez, rowfac;
-    double zhat0 = 0, dzh = 1;
-    std::vector<int> views_y, views_x;
-    // device copies
-    ctlab::ViewParams* d_vp = nullptr;
-    double *d_kap = nullptr, *d_tau = nullptr, *d_colfac = nullptr, *d_exy2 = nullptr;
-    double *d_zhat = nullptr, *d_ez = nullptr, *d_rowfac = nullptr;
-    unsigned char* d_mask = nullptr;
-    int* d_views = nullptr;
-    void* d_cz = nullptr;   // z-prefix integral image (nvox elements)
-    void* d_P = nullptr;    // projection v-prefix (nprj elements)
-    int sat_bits = 64;
-    size_t ws_elem = 0;
-
-    ctlab::DevGeom dev() const
-    {
-        ctlab::DevGeom g;
-        g.XN = in.XN; g.YN = in.YN; g.ZN = in.ZN; g.DNU = in.DNU; g.DNV = in.DNV; g.PN = in.PN;
-        g.dx = in.dx; g.dz = in.dz; g.cone = in.cone;
-        g.vp = d_vp; g.kap = d_kap; g.tau = d_tau; g.colfac = d_colfac; g.exy2 = d_exy2;
-        g.zhat = d_zhat; g.ez = d_ez; g.rowfac = d_rowfac;
-        g.mask = in.mask.empty() ? nullptr : d_mask;
-        g.zhat0 = zhat0; g.dzh = dzh;
-        return g;
-    }
-};
 
 namespace ctlab {
 
@@ -192,6 +146,8 @@ static int build_tables(ctlab_plan* P)
     P->tau.resize((size_t)PN * DNU);
     P->colfac.resize((size_t)PN * DNU);
     P->exy2.resize((size_t)PN * DNU);
+    P->invdk.resize((size_t)PN * DNU);
+    P->esdz.resize((size_t)PN * DNU);
     P->views_y.clear();
     P->views_x.clear();
     for (int p = 0; p < PN; ++p) {
@@ -247,6 +203,8 @@ static int build_tables(ctlab_plan* P)
             const double t = 1.0 / es;
             P->tau[(size_t)p * DNU + u] = t;
             P->exy2[(size_t)p * DNU + u] = ex * ex + ey * ey;
+            P->invdk[(size_t)p * DNU + u] = 1.0 / std::fabs(dk);
+            P->esdz[(size_t)p * DNU + u] = es / P->dzh;
             P->colfac[(size_t)p * DNU + u] =
                 g.cone ? g.dx * g.dx / std::fabs(dk) : g.dx * g.dx * std::fabs(t) / std::fabs(dk);
         }
@@ -265,13 +223,13 @@ static int upload(T** dst, const std::vector<T>& src)
                      "hipMemcpy");
 }
 
-static void free_plan(ctlab_plan* P)
+void free_plan(ctlab_plan* P)
 {
     if (!P) return;
     int cur = 0;
     hipGetDevice(&cur);
     hipSetDevice(P->device);
-    void* ptrs[] = {P->d_vp, P->d_kap, P->d_tau, P->d_colfac, P->d_exy2, P->d_zhat, P->d_ez,
+    void* ptrs[] = {P->d_vp, P->d_kap, P->d_tau, P->d_colfac, P->d_exy2, P->d_invdk, P->d_esdz, P->d_zhat, P->d_ez,
                     P->d_rowfac, P->d_mask, P->d_views, P->d_cz, P->d_P};
     for (void* q : ptrs)
         if (q) hipFree(q);
@@ -293,7 +251,7 @@ static int alloc_workspace(ctlab_plan* P)
     return 0;
 }
 
-static ctlab_plan* finish_plan(ctlab_plan* P, int gpunum)
+ctlab_plan* finish_plan(ctlab_plan* P, int gpunum)
 {
     if (build_tables(P)) { delete P; return nullptr; }
     int ndev = 0;
@@ -308,7 +266,8 @@ static ctlab_plan* finish_plan(ctlab_plan* P, int gpunum)
     std::vector<int> views(P->views_y);
     views.insert(views.end(), P->views_x.begin(), P->views_x.end());
     if (upload(&P->d_vp, P->vp) || upload(&P->d_kap, P->kap) || upload(&P->d_tau, P->tau) ||
-        upload(&P->d_colfac, P->colfac) || upload(&P->d_exy2, P->exy2) || upload(&P->d_zhat, P->zhat) ||
+        upload(&P->d_colfac, P->colfac) || upload(&P->d_exy2, P->exy2) || upload(&P->d_invdk, P->invdk) ||
+        upload(&P->d_esdz, P->esdz) || upload(&P->d_zhat, P->zhat) ||
         upload(&P->d_ez, P->ez) || upload(&P->d_rowfac, P->rowfac) || upload(&P->d_mask, P->in.mask) ||
         upload(&P->d_views, views) || alloc_workspace(P)) {
         free_plan(P);
@@ -370,19 +329,6 @@ ctlab_plan* plan_create_fan(float x0, float y0, const float* xds, const float* y
     if (mask) g.mask.assign(mask, mask + (size_t)XN * YN);
     return finish_plan(P, gpunum);
 }
-
-struct DeviceGuard {
-    int prev = -1;
-    explicit DeviceGuard(int dev)
-    {
-        hipGetDevice(&prev);
-        if (prev != dev) hipSetDevice(dev); else prev = -1;
-    }
-    ~DeviceGuard()
-    {
-        if (prev >= 0) hipSetDevice(prev);
-    }
-};
 
 int plan_proj(ctlab_plan* P, const float* d_vol, float* d_prj, hipStream_t s)
 {

@@ -1,0 +1,83 @@
+// Internal plan object shared by the library's translation units.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <vector>
+
+#include "dd_common.hpp"
+
+namespace ctlab {
+
+struct GeomIn {
+    double x0, y0, z0;
+    int DNU, DNV;
+    std::vector<double> xds, yds, zds, angs, zpos;
+    double cx, cy, cz;
+    int PN, XN, YN, ZN;
+    double dx, dz;
+    std::vector<unsigned char> mask;  // empty = none
+    int cone;
+};
+
+
+template <typename T>
+hipError_t launch_proj(const DevGeom& g, const float* d_vol, void* d_cz, float* d_prj, hipStream_t s);
+template <typename T>
+hipError_t launch_back(const DevGeom& g, const float* d_prj, void* d_P, float* d_vol,
+                       const int* d_views_y, int ny, const int* d_views_x, int nx, hipStream_t s);
+
+}  // namespace ctlab
+
+struct ctlab_plan {
+    int device = 0;
+    ctlab::GeomIn in;
+    std::vector<ctlab::ViewParams> vp;
+    std::vector<double> kap, tau, colfac, exy2, invdk, esdz, zhat, ez, rowfac;
+    double zhat0 = 0, dzh = 1;
+    std::vector<int> views_y, views_x;
+    // device copies
+    ctlab::ViewParams* d_vp = nullptr;
+    double *d_kap = nullptr, *d_tau = nullptr, *d_colfac = nullptr, *d_exy2 = nullptr;
+    double *d_invdk = nullptr, *d_esdz = nullptr;
+    double *d_zhat = nullptr, *d_ez = nullptr, *d_rowfac = nullptr;
+    unsigned char* d_mask = nullptr;
+    int* d_views = nullptr;
+    void* d_cz = nullptr;   // z-prefix integral image (nvox elements)
+    void* d_P = nullptr;    // projection v-prefix (nprj elements)
+    int sat_bits = 64;
+    size_t ws_elem = 0;
+
+    ctlab::DevGeom dev() const
+    {
+        ctlab::DevGeom g;
+        g.XN = in.XN; g.YN = in.YN; g.ZN = in.ZN; g.DNU = in.DNU; g.DNV = in.DNV; g.PN = in.PN;
+        g.dx = in.dx; g.dz = in.dz; g.cone = in.cone;
+        g.vp = d_vp; g.kap = d_kap; g.tau = d_tau; g.colfac = d_colfac; g.exy2 = d_exy2;
+        g.invdk = d_invdk; g.esdz = d_esdz;
+        g.zhat = d_zhat; g.ez = d_ez; g.rowfac = d_rowfac;
+        g.mask = in.mask.empty() ? nullptr : d_mask;
+        g.zhat0 = zhat0; g.dzh = dzh;
+        return g;
+    }
+};
+
+namespace ctlab {
+// builds the device tables of a plan whose `in` is filled; frees P and returns null on error
+ctlab_plan* finish_plan(ctlab_plan* P, int gpunum);
+void free_plan(ctlab_plan* P);
+int plan_proj(ctlab_plan* P, const float* d_vol, float* d_prj, hipStream_t s);
+int plan_back(ctlab_plan* P, const float* d_prj, float* d_vol, hipStream_t s);
+
+struct DeviceGuard {
+    int prev = -1;
+    explicit DeviceGuard(int dev)
+    {
+        hipGetDevice(&prev);
+        if (prev != dev) hipSetDevice(dev); else prev = -1;
+    }
+    ~DeviceGuard()
+    {
+        if (prev >= 0) hipSetDevice(prev);
+    }
+};
+}  // namespace ctlab

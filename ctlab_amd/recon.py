@@ -1,0 +1,121 @@
+"""Device-resident reconstruction (OS-SART, FISTA-TV, CG) over a projector plan.
+
+Mirrors CTLAB's drivers (src/DD3_GPU_recon.h:21-103) and update functors
+(src/SARTWeighting.h, src/FastIterativeShrinkageAlgorithm.h,
+src/utilities.hpp:1062-1268) through libctlab_mi355x.so; every vector is a
+float32 torch tensor in HBM and every kernel runs on the GPU.
+"""
+import ctypes as C
+
+import numpy as np
+
+from . import _lib
+from .dd import Plan, _stream_handle
+
+CG_FR, CG_PRP, CG_CW, CG_DI, CG_DY = range(5)      # CGMethod, src/DD3_GPU_recon.h:14-19
+
+
+def _p(t):
+    return C.c_void_p(t.data_ptr())
+
+
+class Recon:
+    def __init__(self, plan: Plan, osNum=1):
+        self.plan = plan
+        h = _lib.lib().ctlab_recon_create(plan.h, osNum)
+        if not h:
+            _lib.check()
+            raise _lib.CtlabError(-1, "recon creation failed")
+        self.h = C.c_void_p(h)
+        self.osNum = osNum
+
+    def set_data(self, b, stream=None):
+        _lib.check(_lib.lib().ctlab_recon_set_data(self.h, _p(b), _stream_handle(stream)))
+
+    def os_sart(self, x, iters, lam=1.0, nonneg=False, use_graph=True, stream=None):
+        _lib.check(_lib.lib().ctlab_os_sart(self.h, _p(x), iters, lam, int(nonneg), int(use_graph),
+                                            _stream_handle(stream)))
+        return x
+
+    def fista_tv(self, x, iters, tv_lambda, tv_iters=10, L=0.0, nonneg=False, stream=None):
+        obj = np.zeros(max(iters, 1), dtype=np.float64)
+        _lib.check(_lib.lib().ctlab_fista_tv(self.h, _p(x), iters, tv_lambda, tv_iters, L, int(nonneg),
+                                             obj.ctypes.data_as(C.POINTER(C.c_double)),
+                                             _stream_handle(stream)))
+        return obj[:iters]
+
+    def cg(self, x, iters, cgm=CG_FR, stream=None):
+        res = np.zeros(max(iters, 1), dtype=np.float64)
+        _lib.check(_lib.lib().ctlab_cg(self.h, _p(x), iters, cgm, res.ctypes.data_as(C.POINTER(C.c_double)),
+                                       _stream_handle(stream)))
+        return res[:iters]
+
+    def data_fidelity(self, x, stream=None):
+        v = _lib.lib().ctlab_recon_data_fidelity(self.h, _p(x), _stream_handle(stream))
+        _lib.check()
+        return v
+
+    def gradient(self, y, g, stream=None):
+        v = _lib.lib().ctlab_recon_gradient(self.h, _p(y), _p(g), _stream_handle(stream))
+        _lib.check()
+        return v
+
+    def lipschitz(self, iters=20, stream=None):
+        v = _lib.lib().ctlab_recon_lipschitz(self.h, iters, _stream_handle(stream))
+        _lib.check()
+        return v
+
+    def close(self):
+        if getattr(self, "h", None):
+            _lib.lib().ctlab_recon_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+# ---- elementwise kernels (reference functors) ------------------------------
+def sart_prj_weight(prj, real, rowsum, stream=None):
+    _lib.check(_lib.lib().ctlab_sart_prj_weight(_p(prj), _p(real), _p(rowsum), prj.numel(), _stream_handle(stream)))
+    return prj
+
+
+def sart_bak_weight(vol, recon, colsum, stream=None):
+    _lib.check(_lib.lib().ctlab_sart_bak_weight(_p(vol), _p(recon), _p(colsum), vol.numel(), _stream_handle(stream)))
+    return recon
+
+
+def ossart_update(img, cor, weg, msk, lam, stream=None):
+    _lib.check(_lib.lib().ctlab_ossart_update(_p(img), _p(cor), _p(weg), _p(msk), lam, img.numel(),
+                                              _stream_handle(stream)))
+    return img
+
+
+def fista_momentum(cur, las, t1, t2, minV=-2000.0, maxV=6000.0, stream=None):
+    _lib.check(_lib.lib().ctlab_fista_momentum(_p(cur), _p(las), t1, t2, minV, maxV, cur.numel(),
+                                               _stream_handle(stream)))
+    return cur
+
+
+def soft_threshold(x, g, stream=None):
+    _lib.check(_lib.lib().ctlab_soft_threshold(_p(x), g, x.numel(), _stream_handle(stream)))
+    return x
+
+
+def cg_beta(gk, gk1, d, cases, stream=None):
+    out = np.zeros(2, dtype=np.float64)
+    _lib.check(_lib.lib().ctlab_cg_beta(_p(gk), _p(gk1), _p(d), cases, gk.numel(),
+                                        out.ctypes.data_as(C.POINTER(C.c_double)), _stream_handle(stream)))
+    return out[0] / out[1]
+
+
+def tv_prox(b, lam, iters, shape, stream=None):
+    import torch
+    YN, XN, ZN = shape
+    x = torch.empty_like(b)
+    work = torch.empty(7 * b.numel(), dtype=torch.float32, device=b.device)
+    _lib.check(_lib.lib().ctlab_tv_prox(_p(b), _p(x), lam, iters, XN, YN, ZN, _p(work), _stream_handle(stream)))
+    return x

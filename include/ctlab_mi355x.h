@@ -88,6 +88,75 @@ int ctlab_back(ctlab_plan* plan, const float* d_prj, float* d_vol, void* stream)
 int ctlab_plan_views(const ctlab_plan* plan, int* n_yslab, int* n_xslab);
 long long ctlab_plan_workspace_bytes(const ctlab_plan* plan);
 
+/* ---- reconstruction on the device (this library's own) ------------------ */
+typedef struct ctlab_recon ctlab_recon;
+
+/* OS-SART / FISTA-TV / CG state over a plan's geometry: osNum interleaved view
+ * subsets (view p belongs to subset p % osNum), their row sums A_s*1 and column
+ * sums A_s^T*1 (src/DD3_GPU_recon.h:21-28), and work buffers, all in HBM. */
+ctlab_recon* ctlab_recon_create(const ctlab_plan* plan, int osNum);
+void ctlab_recon_destroy(ctlab_recon* recon);
+/* measured projections (device pointer, plan layout, original view order) */
+int ctlab_recon_set_data(ctlab_recon* recon, const float* d_b, void* stream);
+/* OS-SART (src/DD3_GPU_recon.h:60-65, weighting src/SARTWeighting.h:22-91):
+ * x += lambda * A_s^T((b_s - A_s x)/rowSum_s)/colSum_s per subset; nonneg clamps
+ * at 0; use_graph captures one pass over all subsets in a HIP graph and replays it. */
+int ctlab_os_sart(ctlab_recon* recon, float* d_x, int iters, float lambda, int nonneg, int use_graph,
+                  void* stream);
+/* FISTA with the FGP TV prox: x <- prox_{tv_lambda/L TV}(y - A^T(Ay-b)/L), y by the
+ * FISTA momentum (src/FastIterativeShrinkageAlgorithm.h:17-42); L <= 0 estimates
+ * ||A||^2 by power iteration; h_obj (iters, or NULL) receives ||Ay-b||^2. */
+int ctlab_fista_tv(ctlab_recon* recon, float* d_x, int iters, float tv_lambda, int tv_iters, float L,
+                   int nonneg, double* h_obj, void* stream);
+/* CG on the normal equations, beta rule cgm = CG_FR..CG_DY (src/DD3_GPU_recon.h:14-19,
+ * src/utilities.hpp:1228-1268); h_resid (iters, or NULL) receives ||A^T(b-Ax)||^2. */
+int ctlab_cg(ctlab_recon* recon, float* d_x, int iters, int cgm, double* h_resid, void* stream);
+/* ||Ax - b||^2 (FP64) */
+double ctlab_recon_data_fidelity(ctlab_recon* recon, const float* d_x, void* stream);
+/* d_g = A^T(Ay - b); returns ||Ay - b||^2 */
+double ctlab_recon_gradient(ctlab_recon* recon, const float* d_y, float* d_g, void* stream);
+/* largest eigenvalue of A^T A (power iteration) */
+double ctlab_recon_lipschitz(ctlab_recon* recon, int iters, void* stream);
+
+/* elementwise / reduction kernels on device buffers (stream-ordered):
+ * SART_Weighting::Proj / Back (src/SARTWeighting.h:97-136), ossart_update_functor
+ * (src/utilities.hpp:1202-1223), FISTA_functor (src/FastIterativeShrinkageAlgorithm.h:
+ * 17-42), _softThreshold_functor (src/utilities.hpp:1097-1117), calculateBetaForCG
+ * (src/utilities.hpp:1611-1621; h_numden receives the two FP64 sums). */
+int ctlab_sart_prj_weight(float* d_prj, const float* d_real, const float* d_rowSum, long long n, void* stream);
+int ctlab_sart_bak_weight(const float* d_vol, float* d_recon, const float* d_colSum, long long n, void* stream);
+int ctlab_ossart_update(float* d_img, const float* d_cor, const float* d_weg, const float* d_msk, float lambda,
+                        long long n, void* stream);
+int ctlab_fista_momentum(float* d_cur, const float* d_las, float t1, float t2, float minV, float maxV,
+                         long long n, void* stream);
+int ctlab_soft_threshold(float* d_x, float g, long long n, void* stream);
+int ctlab_cg_beta(const float* d_gk, const float* d_gk1, const float* d_d, int cases, long long n,
+                  double* h_numden, void* stream);
+/* isotropic TV prox (FGP, iters steps) on a (Z,X,Y) volume; d_work: 7*XN*YN*ZN floats */
+int ctlab_tv_prox(const float* d_b, float* d_x, float lam, int iters, int XN, int YN, int ZN, float* d_work,
+                  void* stream);
+
+/* ---- learned proximal CNN (no counterpart in CTLAB) ----------------------
+ * 5 layers of 3x3x3 convolution 1 -> 32 -> 32 -> 32 -> 32 -> 1, ReLU after
+ * layers 1-4, output x + net(x); BF16 weights / activations on MFMA with FP32
+ * accumulation.  Parameters: one flat FP32 vector W1 b1 W2 b2 ... W5 b5 with
+ * W_l[27][cout][cin], tap t = (dy+1)*9 + (dx+1)*3 + (dz+1).  An object is sized
+ * for one volume shape; training needs ZN % 8 == 0. */
+typedef struct ctlab_cnn ctlab_cnn;
+int ctlab_cnn_num_params(void);
+ctlab_cnn* ctlab_cnn_create(int XN, int YN, int ZN, int training, int gpunum);
+void ctlab_cnn_destroy(ctlab_cnn* cnn);
+int ctlab_cnn_set_params(ctlab_cnn* cnn, const float* h_params);
+int ctlab_cnn_get_params(ctlab_cnn* cnn, float* h_params);
+/* d_out = d_x + net(d_x) (device pointers, volume layout) */
+int ctlab_cnn_forward(ctlab_cnn* cnn, const float* d_x, float* d_out, void* stream);
+/* forward + backward of 0.5*mean((x + net(x) - target)^2); returns the loss,
+ * keeps the gradient on the device (copied to h_grad when non-NULL) */
+double ctlab_cnn_loss_grad(ctlab_cnn* cnn, const float* d_x, const float* d_target, float* d_out, float* h_grad,
+                           void* stream);
+/* one Adam step with the gradient of the last ctlab_cnn_loss_grad */
+int ctlab_cnn_adam_step(ctlab_cnn* cnn, float lr, float beta1, float beta2, float eps, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -83,8 +83,8 @@ def _declare(L):
 
 def _declare_conv(L):
     L.co_conv3d_fwd.argtypes = [_dp, _dp, _dp, _dp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int]
-    L.co_conv3d_dgrad.argtypes = [_dp, _dp, _dp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int]
-    L.co_conv3d_wgrad.argtypes = [_dp, _dp, _dp, _dp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int]
+    L.co_conv3d_dgrad.argtypes = [_dp, _dp, _dp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int]
+    L.co_conv3d_wgrad.argtypes = [_dp, _dp, _dp, _dp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int]
 
 
 def _d(a):
@@ -257,3 +257,35 @@ def tv_prox(b, XN, YN, ZN, lam, iters):
 def tv_value(x, XN, YN, ZN):
     x, p = _d(x.reshape(-1))
     return lib().ro_tv_value(p, XN, YN, ZN)
+
+
+# ---- 3x3x3 convolution (learned prox CNN checker) ---------------------------
+def conv3d(x, w, b, shape):
+    """x: (nvox, cin), w: (27, cout, cin), b: (cout,) or None; shape = (YN, XN, ZN)."""
+    YN, XN, ZN = shape
+    x, xp = _d(x); w, wp = _d(w)
+    cout, cin = w.shape[1], w.shape[2]
+    out = np.zeros((x.shape[0], cout))
+    bp = None
+    if b is not None:
+        b, bp = _d(b)
+    lib().co_conv3d_fwd(xp, wp, bp, out.ctypes.data_as(_dp), XN, YN, ZN, cin, cout, 0)
+    return out
+
+
+def conv3d_dgrad(gout, w, shape):
+    YN, XN, ZN = shape
+    gout, gp = _d(gout); w, wp = _d(w)
+    cout, cin = w.shape[1], w.shape[2]
+    gin = np.zeros((gout.shape[0], cin))
+    lib().co_conv3d_dgrad(gp, wp, gin.ctypes.data_as(_dp), XN, YN, ZN, cin, cout)
+    return gin
+
+
+def conv3d_wgrad(x, gout, cin, cout, shape):
+    YN, XN, ZN = shape
+    x, xp = _d(x); gout, gp = _d(gout)
+    gw = np.zeros((27, cout, cin))
+    gb = np.zeros(cout)
+    lib().co_conv3d_wgrad(xp, gp, gw.ctypes.data_as(_dp), gb.ctypes.data_as(_dp), XN, YN, ZN, cin, cout)
+    return gw, gb

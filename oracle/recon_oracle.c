@@ -20,16 +20,20 @@
 #include <stdlib.h>
 #include <string.h>
 
-/* r = (b - Ax) / rowSum, 0 where rowSum <= 1e-7 */
+/* SART_Weighting's epsilon is `static const float epsilon = 1.0e-7`
+ * (src/SARTWeighting.h:138): the guard compares against 1e-7 rounded to float. */
+static const double kSartEps = (double)1.0e-7f;
+
+/* r = (b - Ax) / rowSum, 0 where rowSum <= eps */
 void ro_sart_prj_weight(double *prj, const double *realPrj, const double *rowSum, long n)
 {
-    for (long i = 0; i < n; ++i) prj[i] = rowSum[i] > 1.0e-7 ? (realPrj[i] - prj[i]) / rowSum[i] : 0.0;
+    for (long i = 0; i < n; ++i) prj[i] = rowSum[i] > kSartEps ? (realPrj[i] - prj[i]) / rowSum[i] : 0.0;
 }
 
-/* recon += vol / colSum, 0 where colSum <= 1e-7 */
+/* recon += vol / colSum, 0 where colSum <= eps */
 void ro_sart_bak_weight(const double *vol, double *recon, const double *colSum, long n)
 {
-    for (long i = 0; i < n; ++i) recon[i] += colSum[i] > 1.0e-7 ? vol[i] / colSum[i] : 0.0;
+    for (long i = 0; i < n; ++i) recon[i] += colSum[i] > kSartEps ? vol[i] / colSum[i] : 0.0;
 }
 
 /* img = (img + lambda*cor/weg)*msk, 0 where |weg| < 1e-9 */
