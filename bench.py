@@ -81,6 +81,7 @@ def main():
     ap.add_argument("--sat-bits", type=int, default=64)
     ap.add_argument("--cpu-views", type=int, default=16)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-tune", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the multi-GPU "
@@ -98,6 +99,11 @@ def main():
     prj = torch.empty(g.prj_shape, dtype=torch.float32, device=dev)
     bak = torch.empty(g.vol_shape, dtype=torch.float32, device=dev)
     stream = torch.cuda.current_stream()
+    # tile autotuner on scratch copies (setup, not timed)
+    tune = None
+    if not args.no_tune:
+        tune = plan.autotune(vol.clone(), torch.empty_like(prj))
+        torch.cuda.synchronize()
 
     for _ in range(args.warmup):
         plan.proj(vol, out=prj)
@@ -142,7 +148,8 @@ def main():
                                "volume (0.5x0.5x1.0 mm), 984 views, 888x64 arc detector "
                                "(1.0239x1.0963 mm), SID 541, SDD 949",
                    "XN": g.XN, "YN": g.YN, "ZN": g.ZN, "PN": g.PN, "DNU": g.DNU, "DNV": g.DNV,
-                   "sat_bits": args.sat_bits},
+                   "sat_bits": args.sat_bits, "tiles": {"nu": plan.tiles()[0], "nm": plan.tiles()[1]}},
+        "autotune_ms": tune,
         "roofline": {"bound": "hbm", "achieved": ach / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
                      "frac": ach / HBM_PEAK,
                      "traffic": (traffic or {}).get(dominant + "_bytes_per_call"),

@@ -60,6 +60,12 @@ def _declare(L):
     L.ctlab_proj.restype = i
     L.ctlab_back.argtypes = [_vp, _vp, _vp, _vp]
     L.ctlab_back.restype = i
+    L.ctlab_plan_set_tiles.argtypes = [_vp, i, i]
+    L.ctlab_plan_set_tiles.restype = i
+    L.ctlab_plan_get_tiles.argtypes = [_vp, _ip, _ip]
+    L.ctlab_plan_get_tiles.restype = i
+    L.ctlab_plan_autotune.argtypes = [_vp, _vp, _vp, C.POINTER(C.c_float)]
+    L.ctlab_plan_autotune.restype = i
     L.ctlab_plan_views.argtypes = [_vp, _ip, _ip]
     L.ctlab_plan_views.restype = i
     L.ctlab_plan_workspace_bytes.argtypes = [_vp]

@@ -44,6 +44,8 @@ struct DevGeom {
     const double* rowfac;     // DNV: 1/(zhat[v+1]-zhat[v]) (cone) or 1 (fan)
     const unsigned char* mask;// XN*YN or nullptr
     double zhat0, dzh;        // uniform row model used by the backprojector
+    int tile_nu;              // projector: detector columns per wavefront (8, 16, 32)
+    int tile_nm;              // backprojector: voxel rows per wavefront (8, 16, 32)
 };
 
 }  // namespace ctlab

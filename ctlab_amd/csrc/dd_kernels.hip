@@ -28,14 +28,19 @@
 
 namespace ctlab {
 
-// Intra-wavefront hand-off through LDS: every lane writes, then every lane
-// reads other lanes' values.  LDS executes one wavefront's instructions in
-// order; the fences keep the compiler from moving the accesses.
-__device__ __forceinline__ void wave_lds_sync()
+// Intra-wavefront hand-off through LDS (every lane writes its prefix entry,
+// then reads two entries written by other lanes): the LDS executes one
+// wavefront's instructions in issue order, and the compiler keeps the store /
+// load order because the indices may alias, so no fence is needed (a fence
+// would also wait for the prefetched global loads still in flight).
+
+// uniform read of a double held by lane l
+__device__ __forceinline__ double readlane_d(double v, int l)
 {
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_readlane((int)(b & 0xffffffffLL), l);
+    const int hi = __builtin_amdgcn_readlane((int)(b >> 32), l);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
 }
 
 // piecewise-linear read of a prefix table G[0..n] (G[0] = 0) at b in [0, n]
@@ -53,21 +58,32 @@ __device__ __forceinline__ T lds_interp(const T* G, double b, int n)
 // ---------------------------------------------------------------------------
 // z-prefix of every masked voxel column: cz[col*ZN + k] = sum_{k'<=k} f*mask
 // ---------------------------------------------------------------------------
+// inclusive prefix over the 64 lanes of a wavefront (lane l gets sum of lanes 0..l)
+template <typename T>
+__device__ __forceinline__ T wave_inclusive_scan(T v, int lane)
+{
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const T n = __shfl_up(v, o, 64);
+        if (lane >= o) v += n;
+    }
+    return v;
+}
+
+// One wavefront per voxel column (lane = z; ZN <= 64): coalesced read of the
+// column, wavefront scan, coalesced write.
 template <typename T>
 __global__ void __launch_bounds__(256)
 zprefix_kernel(const float* __restrict__ vol, T* __restrict__ cz,
                const unsigned char* __restrict__ mask, long ncol, int ZN)
 {
-    const long col = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    const int lane = threadIdx.x & 63;
+    const long col = blockIdx.x * 4L + (threadIdx.x >> 6);
     if (col >= ncol) return;
     const T mk = (mask == nullptr || mask[col]) ? T(1) : T(0);
-    const float* __restrict__ src = vol + col * ZN;
-    T* __restrict__ dst = cz + col * ZN;
-    T run = T(0);
-    for (int k = 0; k < ZN; ++k) {
-        run += (T)src[k] * mk;
-        dst[k] = run;
-    }
+    const T v = lane < ZN ? (T)vol[col * ZN + lane] * mk : T(0);
+    const T r = wave_inclusive_scan(v, lane);
+    if (lane < ZN) cz[col * ZN + lane] = r;
 }
 
 // ---------------------------------------------------------------------------
@@ -88,10 +104,12 @@ dd_proj_walk(DevGeom g, const T* __restrict__ cz, float* __restrict__ prj)
     T* G = sG[wv];
     if (lane == 0) G[0] = T(0);
     const ViewParams vp = g.vp[p];
-    const double* __restrict__ kap = g.kap + (long)p * (g.DNU + 1);
-    const double* __restrict__ tau = g.tau + (long)p * g.DNU;
     const int u0 = blockIdx.x * NU;
     const int nu = min(NU, g.DNU - u0);
+    // this tile's boundary slopes (lanes 0..nu) and magnifications (lanes 0..nu-1)
+    // are the same for every slab: load once, read per column with v_readlane
+    const double kap_l = g.kap[(long)p * (g.DNU + 1) + u0 + min(lane, nu)];
+    const double tau_l = g.tau[(long)p * g.DNU + u0 + min(lane, nu - 1)];
     const int ZN = g.ZN, NR = vp.NR, DNV = g.DNV;
     const int zl = min(lane, ZN - 1);
     const int vl = min(lane, DNV - 1);
@@ -108,8 +126,8 @@ dd_proj_walk(DevGeom g, const T* __restrict__ cz, float* __restrict__ prj)
     for (int j = 0; j < vp.NS; ++j) {
         const double sj = vp.s0 + j * g.dx;
         const double rs = sj * inv_dx;
-        const double afirst = vp.rbase + rs * kap[vp.asc ? u0 : u0 + nu];
-        const double alast = vp.rbase + rs * kap[vp.asc ? u0 + nu : u0];
+        const double afirst = vp.rbase + rs * readlane_d(kap_l, vp.asc ? 0 : nu);
+        const double alast = vp.rbase + rs * readlane_d(kap_l, vp.asc ? nu : 0);
         if (alast <= 0.0 || afirst >= (double)NR) continue;
         const T qj = (T)(g.cone ? 1.0 / (sj * sj) : 1.0 / fabs(sj));
         const T* __restrict__ base = cz + j * slab_off + zl;
@@ -124,7 +142,7 @@ dd_proj_walk(DevGeom g, const T* __restrict__ cz, float* __restrict__ prj)
         T S = T(0);
         T Eprev = (T)(ap - mf) * c0;
         for (int q = 1; q <= nu; ++q) {
-            double aq = vp.rbase + rs * kap[vp.asc ? u0 + q : u0 + nu - q];
+            double aq = vp.rbase + rs * readlane_d(kap_l, vp.asc ? q : nu - q);
             aq = fmin(fmax(aq, 0.0), (double)NR);
             while (m < NR - 1 && aq >= mf + 1.0) {
                 S += c0;
@@ -139,14 +157,11 @@ dd_proj_walk(DevGeom g, const T* __restrict__ cz, float* __restrict__ prj)
             const T H = E - Eprev;
             Eprev = E;
             if (aq > ap) {
-                const int u = vp.asc ? u0 + q - 1 : u0 + nu - q;
-                const double t = g.cone ? sj * tau[u] : 1.0;
+                const double t = g.cone ? sj * readlane_d(tau_l, vp.asc ? q - 1 : nu - q) : 1.0;
                 G[lane + 1] = H;
-                wave_lds_sync();
                 const T Dhi = lds_interp(G, vp.beta0 + t * zhi, ZN);
                 const T Dlo = lds_interp(G, vp.beta0 + t * zlo, ZN);
                 acc[q - 1] += (Dhi - Dlo) * qj;
-                wave_lds_sync();
             }
             ap = aq;
         }
@@ -174,35 +189,35 @@ template <typename T>
 __global__ void __launch_bounds__(256)
 prjprefix_kernel(DevGeom g, const float* __restrict__ prj, T* __restrict__ P)
 {
-    const long idx = blockIdx.x * (long)blockDim.x + threadIdx.x;  // p*DNU + u
+    const int lane = threadIdx.x & 63;
+    const long idx = blockIdx.x * 4L + (threadIdx.x >> 6);  // p*DNU + u, one wavefront each (DNV <= 64)
     if (idx >= (long)g.PN * g.DNU) return;
-    const double ta = fabs(g.tau[idx]);
-    const double e2 = g.exy2[idx];
-    const float* __restrict__ src = prj + idx * g.DNV;
-    T* __restrict__ dst = P + idx * g.DNV;
-    T run = T(0);
-    for (int v = 0; v < g.DNV; ++v) {
-        const double ezv = g.ez[v];
-        const double L = g.dx * sqrt(e2 + ezv * ezv) * ta;
-        run += (T)((double)src[v] * L);
-        dst[v] = run;
+    const int DNV = g.DNV;
+    T v = T(0);
+    if (lane < DNV) {
+        const double ezv = g.ez[lane];
+        const double L = g.dx * sqrt(g.exy2[idx] + ezv * ezv) * fabs(g.tau[idx]);
+        v = (T)((double)prj[idx * DNV + lane] * L);
     }
+    const T r = wave_inclusive_scan(v, lane);
+    if (lane < DNV) P[idx * DNV + lane] = r;
 }
 
-// Adjoint z-part for detector column u: sum_v y*L * (fraction of cell v covered
-// by the voxel z-interval [lane, lane+1] mapped onto the detector rows), read
-// from the column's v-prefix Pu (lane v holds prefix v+1) through LDS.
+// Adjoint z-part for one detector column: sum_v y*L * (fraction of cell v
+// covered by the voxel z-interval [lane, lane+1] mapped onto the detector rows),
+// read from the column's v-prefix Pu (lane v holds prefix v+1) through LDS.
+// esdz = (cell-centre slab coordinate - source) / row pitch, so that the
+// voxel boundary b maps to the fractional row (b - beta0) * esdz / sj - zhat0/dzh.
 template <typename T>
-__device__ __forceinline__ T column_delta(const DevGeom& g, T* G, T Pu, int p, int u, double inv_sj,
+__device__ __forceinline__ T column_delta(const DevGeom& g, T* G, T Pu, double esdz, double inv_sj,
                                           double beta0, int lane)
 {
     const int DNV = g.DNV;
     G[lane + 1] = Pu;
-    wave_lds_sync();
     double nlo, nhi;
     T sgn = T(1);
     if (g.cone) {
-        const double inv = inv_sj * g.esdz[(long)p * g.DNU + u];   // 1 / (t * dzh)
+        const double inv = inv_sj * esdz;   // 1 / (t * dzh)
         const double off = -beta0 * inv - g.zhat0 / g.dzh;
         nlo = (double)lane * inv + off;
         nhi = nlo + inv;
@@ -211,19 +226,52 @@ __device__ __forceinline__ T column_delta(const DevGeom& g, T* G, T Pu, int p, i
         nlo = (double)lane;
         nhi = (double)(lane + 1);
     }
-    const T r = (lds_interp(G, nhi, DNV) - lds_interp(G, nlo, DNV)) * sgn;
-    wave_lds_sync();
-    return r;
+    return (lds_interp(G, nhi, DNV) - lds_interp(G, nlo, DNV)) * sgn;
+}
+
+// Column record in walk order (increasing row coordinate a):
+//   x = slope of the lower boundary, y = slope of the upper boundary,
+//   z = 1/|slope step|, w = esdz of the column.
+struct ColRec { double klo, khi, invdk, esdz; };
+
+// First walk-order column whose upper boundary lies above row r0 = 8*r8, per
+// (view, slab, r8); 0xFFFF when every column lies below r0.
+__global__ void __launch_bounds__(256)
+back_start_kernel(DevGeom g, const ColRec* __restrict__ rec, unsigned short* __restrict__ start, int SJ, int SR)
+{
+    const long idx = blockIdx.x * 256L + threadIdx.x;
+    if (idx >= (long)g.PN * SJ * SR) return;
+    const int r8 = (int)(idx % SR);
+    const long pj = idx / SR;
+    const int j = (int)(pj % SJ);
+    const int p = (int)(pj / SJ);
+    const ViewParams vp = g.vp[p];
+    const int DNU = g.DNU;
+    const double r0 = 8.0 * r8;
+    if (j >= vp.NS || r0 >= vp.NR) { start[idx] = 0xFFFF; return; }
+    const double rs = (vp.s0 + j * g.dx) / g.dx;
+    const ColRec* R = rec + (long)p * DNU;
+    auto A = [&](int q) { return vp.rbase + rs * (q < DNU ? R[q].klo : R[DNU - 1].khi); };
+    if (A(DNU) <= r0) { start[idx] = 0xFFFF; return; }
+    int lo = 1, hi = DNU;
+    while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if (A(mid) > r0) hi = mid; else lo = mid + 1;
+    }
+    start[idx] = (unsigned short)(lo - 1);
 }
 
 // Block = 4 wavefronts = 4 consecutive slabs of one tile of NM rows (their
 // detector-column ranges nearly coincide); each wavefront loops over the views
-// of one slab orientation; lane = z.
+// of one slab orientation; lane = z.  Per view: the start column comes from
+// the table above (prefetched one view ahead), then columns are consumed in
+// walk order with the next column's record and P row in flight.
 template <typename T, int NM>
 __global__ void __launch_bounds__(256)
 dd_back_walk(DevGeom g, const T* __restrict__ P, float* __restrict__ vol,
              const int* __restrict__ views, int nviews, int yslab, int accumulate,
-             int NS, int NR, int ntr)
+             int NS, int NR, int ntr, const ColRec* __restrict__ rec,
+             const unsigned short* __restrict__ start, int SJ, int SR)
 {
     __shared__ T sG[4][72];
     const int lane = threadIdx.x & 63;
@@ -238,38 +286,41 @@ dd_back_walk(DevGeom g, const T* __restrict__ P, float* __restrict__ vol,
     const int ZN = g.ZN, DNU = g.DNU, DNV = g.DNV;
     const int vl = min(lane, DNV - 1);
     const double inv_dx = 1.0 / g.dx;
+    const long srow = (long)j * SR + (r0 >> 3);
 
     T acc[NM];
 #pragma unroll
     for (int i = 0; i < NM; ++i) acc[i] = T(0);
 
+    int pn = nviews > 0 ? views[0] : 0;
+    int stn = nviews > 0 ? start[(long)pn * SJ * SR + srow] : 0xFFFF;
     for (int iv = 0; iv < nviews; ++iv) {
-        const int p = views[iv];
+        const int p = __builtin_amdgcn_readfirstlane(pn);
+        const int st = __builtin_amdgcn_readfirstlane(stn);
+        if (iv + 1 < nviews) {
+            pn = views[iv + 1];
+            stn = start[(long)pn * SJ * SR + srow];
+        }
+        if (st == 0xFFFF) continue;
         const ViewParams vp = g.vp[p];
-        const double* __restrict__ kap = g.kap + (long)p * (DNU + 1);
-        const double* __restrict__ invdk = g.invdk + (long)p * DNU;
-        const T* __restrict__ Pv = P + (long)p * DNU * DNV + vl;
         const double sj = vp.s0 + j * g.dx;
         const double rs = sj * inv_dx;
-        const int asc = vp.asc;
-        auto A = [&](int q) { return vp.rbase + rs * kap[asc ? q : DNU - q]; };
-        const double A0 = A(0), AN = A(DNU);
-        if (AN <= (double)r0 || A0 >= (double)(r0 + nm)) continue;
-        int lo = 1, hi = DNU;
-        while (lo < hi) {
-            const int mid = (lo + hi) >> 1;
-            if (A(mid) > (double)r0) hi = mid; else lo = mid + 1;
-        }
-        int c = lo - 1;
         const double inv_sj = 1.0 / sj;
         const double inv_rs = fabs(g.dx * inv_sj);
-        auto ucol = [&](int cc) { return asc ? cc : DNU - 1 - cc; };
-        // P rows of columns c and c+1 in flight
-        T Pr0 = Pv[(long)ucol(c) * DNV];
-        T Pr1 = Pv[(long)ucol(min(c + 1, DNU - 1)) * DNV];
-        double alo = A(c), ahi = A(c + 1);
-        double iw = inv_rs * invdk[ucol(c)];
-        T D = column_delta(g, G, Pr0, p, ucol(c), inv_sj, vp.beta0, lane);
+        const int asc = vp.asc;
+        const ColRec* __restrict__ R = rec + (long)p * DNU;
+        const T* __restrict__ Pv = P + (long)p * DNU * DNV + vl;
+        int c = st;
+        ColRec r = R[c];
+        double alo = vp.rbase + rs * r.klo;
+        if (alo >= (double)(r0 + nm)) continue;
+        double ahi = vp.rbase + rs * r.khi;
+        double iw = inv_rs * r.invdk;
+        const int c1 = min(c + 1, DNU - 1);
+        T Pr0 = Pv[(long)(asc ? c : DNU - 1 - c) * DNV];
+        ColRec rn = R[c1];
+        T Pr1 = Pv[(long)(asc ? c1 : DNU - 1 - c1) * DNV];
+        T D = column_delta(g, G, Pr0, r.esdz, inv_sj, vp.beta0, lane);
         T Tsum = T(0), Wprev = T(0);
 #pragma unroll
         for (int i = 0; i <= NM; ++i) {
@@ -279,11 +330,12 @@ dd_back_walk(DevGeom g, const T* __restrict__ P, float* __restrict__ vol,
                 Tsum += D;
                 ++c;
                 alo = ahi;
-                ahi = A(c + 1);
-                Pr0 = Pr1;
-                Pr1 = Pv[(long)ucol(min(c + 1, DNU - 1)) * DNV];
-                iw = inv_rs * invdk[ucol(c)];
-                D = column_delta(g, G, Pr0, p, ucol(c), inv_sj, vp.beta0, lane);
+                ahi = vp.rbase + rs * rn.khi;
+                iw = inv_rs * rn.invdk;
+                D = column_delta(g, G, Pr1, rn.esdz, inv_sj, vp.beta0, lane);
+                const int cn = min(c + 1, DNU - 1);
+                rn = R[cn];
+                Pr1 = Pv[(long)(asc ? cn : DNU - 1 - cn) * DNV];
             }
             const double fr = fmin(fmax((mp - alo) * iw, 0.0), 1.0);
             const T W = Tsum + (T)fr * D;
@@ -309,41 +361,62 @@ dd_back_walk(DevGeom g, const T* __restrict__ P, float* __restrict__ vol,
 // ---------------------------------------------------------------------------
 // launchers
 // ---------------------------------------------------------------------------
-constexpr int kProjNU = 16;
-constexpr int kBackNM = 16;
+template <typename T, int NU>
+static void launch_proj_walk(const DevGeom& g, const T* cz, float* d_prj, hipStream_t s)
+{
+    const int ntiles = (g.DNU + NU - 1) / NU;
+    hipLaunchKernelGGL((dd_proj_walk<T, NU>), dim3(ntiles, (g.PN + 3) / 4), dim3(256), 0, s, g, cz, d_prj);
+}
+
+template <typename T, int NM>
+static void launch_back_walk(const DevGeom& g, const T* P, float* d_vol, const int* views, int nv, int yslab,
+                             int acc, const ColRec* rec, const unsigned short* start, int SJ, int SR, hipStream_t s)
+{
+    const int NS = yslab ? g.YN : g.XN, NR = yslab ? g.XN : g.YN, ntr = (NR + NM - 1) / NM;
+    hipLaunchKernelGGL((dd_back_walk<T, NM>), dim3((unsigned)(ntr * ((NS + 3) / 4))), dim3(256), 0, s, g, P, d_vol,
+                       views, nv, yslab, acc, NS, NR, ntr, rec, start, SJ, SR);
+}
 
 template <typename T>
 hipError_t launch_proj(const DevGeom& g, const float* d_vol, void* d_cz, float* d_prj, hipStream_t s)
 {
     T* cz = static_cast<T*>(d_cz);
     const long ncol = (long)g.XN * g.YN;
-    hipLaunchKernelGGL(zprefix_kernel<T>, dim3((unsigned)((ncol + 255) / 256)), dim3(256), 0, s,
+    hipLaunchKernelGGL(zprefix_kernel<T>, dim3((unsigned)((ncol + 3) / 4)), dim3(256), 0, s,
                        d_vol, cz, g.mask, ncol, g.ZN);
-    const int ntiles = (g.DNU + kProjNU - 1) / kProjNU;
-    hipLaunchKernelGGL((dd_proj_walk<T, kProjNU>), dim3(ntiles, (g.PN + 3) / 4), dim3(256), 0, s,
-                       g, cz, d_prj);
+    if (g.tile_nu == 32) launch_proj_walk<T, 32>(g, cz, d_prj, s);
+    else if (g.tile_nu == 8) launch_proj_walk<T, 8>(g, cz, d_prj, s);
+    else launch_proj_walk<T, 16>(g, cz, d_prj, s);
+    return hipGetLastError();
+}
+
+hipError_t launch_back_start(const DevGeom& g, const void* d_rec, unsigned short* d_start, int SJ, int SR,
+                             hipStream_t s)
+{
+    const long n = (long)g.PN * SJ * SR;
+    hipLaunchKernelGGL(back_start_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, g,
+                       (const ColRec*)d_rec, d_start, SJ, SR);
     return hipGetLastError();
 }
 
 template <typename T>
 hipError_t launch_back(const DevGeom& g, const float* d_prj, void* d_P, float* d_vol,
-                       const int* d_views_y, int ny, const int* d_views_x, int nx, hipStream_t s)
+                       const int* d_views_y, int ny, const int* d_views_x, int nx, const void* d_rec,
+                       const unsigned short* d_start, int SJ, int SR, hipStream_t s)
 {
+    const ColRec* rec = (const ColRec*)d_rec;
     T* P = static_cast<T*>(d_P);
     const long ncu = (long)g.PN * g.DNU;
-    hipLaunchKernelGGL(prjprefix_kernel<T>, dim3((unsigned)((ncu + 255) / 256)), dim3(256), 0, s,
+    hipLaunchKernelGGL(prjprefix_kernel<T>, dim3((unsigned)((ncu + 3) / 4)), dim3(256), 0, s,
                        g, d_prj, P);
-    // pass 1: y-slab views (slab = y, rows along x); overwrites
-    {
-        const int NS = g.YN, NR = g.XN, ntr = (NR + kBackNM - 1) / kBackNM;
-        hipLaunchKernelGGL((dd_back_walk<T, kBackNM>), dim3((unsigned)(ntr * ((NS + 3) / 4))), dim3(256), 0,
-                           s, g, P, d_vol, d_views_y, ny, 1, 0, NS, NR, ntr);
-    }
-    // pass 2: x-slab views (slab = x, rows along y); accumulates
-    {
-        const int NS = g.XN, NR = g.YN, ntr = (NR + kBackNM - 1) / kBackNM;
-        hipLaunchKernelGGL((dd_back_walk<T, kBackNM>), dim3((unsigned)(ntr * ((NS + 3) / 4))), dim3(256), 0,
-                           s, g, P, d_vol, d_views_x, nx, 0, 1, NS, NR, ntr);
+    // pass 1: y-slab views (slab = y, rows along x) overwrite; pass 2: x-slab views accumulate
+    for (int pass = 0; pass < 2; ++pass) {
+        const int ys = pass == 0;
+        const int* vw = ys ? d_views_y : d_views_x;
+        const int nv = ys ? ny : nx;
+        if (g.tile_nm == 32) launch_back_walk<T, 32>(g, P, d_vol, vw, nv, ys, pass, rec, d_start, SJ, SR, s);
+        else if (g.tile_nm == 8) launch_back_walk<T, 8>(g, P, d_vol, vw, nv, ys, pass, rec, d_start, SJ, SR, s);
+        else launch_back_walk<T, 16>(g, P, d_vol, vw, nv, ys, pass, rec, d_start, SJ, SR, s);
     }
     return hipGetLastError();
 }
@@ -351,8 +424,8 @@ hipError_t launch_back(const DevGeom& g, const float* d_prj, void* d_P, float* d
 template hipError_t launch_proj<double>(const DevGeom&, const float*, void*, float*, hipStream_t);
 template hipError_t launch_proj<float>(const DevGeom&, const float*, void*, float*, hipStream_t);
 template hipError_t launch_back<double>(const DevGeom&, const float*, void*, float*, const int*, int,
-                                        const int*, int, hipStream_t);
+                                        const int*, int, const void*, const unsigned short*, int, int, hipStream_t);
 template hipError_t launch_back<float>(const DevGeom&, const float*, void*, float*, const int*, int,
-                                       const int*, int, hipStream_t);
+                                       const int*, int, const void*, const unsigned short*, int, int, hipStream_t);
 
 }  // namespace ctlab

@@ -3,6 +3,7 @@
 // DD3Proj_gpu / DD3Back_gpu / DD2_multiGPU as thin H2D -> plan -> D2H wrappers.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
@@ -148,6 +149,11 @@ static int build_tables(ctlab_plan* P)
     P->exy2.resize((size_t)PN * DNU);
     P->invdk.resize((size_t)PN * DNU);
     P->esdz.resize((size_t)PN * DNU);
+    P->rec.resize((size_t)PN * DNU * 4);
+    if (DNU >= 0xFFFF) {
+        set_error(CTLAB_ERR_GEOMETRY, "DNU (%d) must be below 65535", DNU);
+        return -1;
+    }
     P->views_y.clear();
     P->views_x.clear();
     for (int p = 0; p < PN; ++p) {
@@ -210,6 +216,15 @@ static int build_tables(ctlab_plan* P)
         }
         V.asc = (V.s0 * dk0) > 0;
         (yslab ? P->views_y : P->views_x).push_back(p);
+        // walk-order (increasing row coordinate) column records for the backprojector
+        for (int c = 0; c < DNU; ++c) {
+            const int u = V.asc ? c : DNU - 1 - c;
+            double* r = &P->rec[((size_t)p * DNU + c) * 4];
+            r[0] = V.asc ? kp[u] : kp[u + 1];
+            r[1] = V.asc ? kp[u + 1] : kp[u];
+            r[2] = P->invdk[(size_t)p * DNU + u];
+            r[3] = P->esdz[(size_t)p * DNU + u];
+        }
     }
     return 0;
 }
@@ -229,7 +244,7 @@ void free_plan(ctlab_plan* P)
     int cur = 0;
     hipGetDevice(&cur);
     hipSetDevice(P->device);
-    void* ptrs[] = {P->d_vp, P->d_kap, P->d_tau, P->d_colfac, P->d_exy2, P->d_invdk, P->d_esdz, P->d_zhat, P->d_ez,
+    void* ptrs[] = {P->d_vp, P->d_kap, P->d_tau, P->d_colfac, P->d_exy2, P->d_invdk, P->d_esdz, P->d_zhat, P->d_ez, P->d_rec, P->d_start,
                     P->d_rowfac, P->d_mask, P->d_views, P->d_cz, P->d_P};
     for (void* q : ptrs)
         if (q) hipFree(q);
@@ -269,10 +284,22 @@ ctlab_plan* finish_plan(ctlab_plan* P, int gpunum)
         upload(&P->d_colfac, P->colfac) || upload(&P->d_exy2, P->exy2) || upload(&P->d_invdk, P->invdk) ||
         upload(&P->d_esdz, P->esdz) || upload(&P->d_zhat, P->zhat) ||
         upload(&P->d_ez, P->ez) || upload(&P->d_rowfac, P->rowfac) || upload(&P->d_mask, P->in.mask) ||
-        upload(&P->d_views, views) || alloc_workspace(P)) {
+        upload(&P->d_views, views) || upload(&P->d_rec, P->rec) || alloc_workspace(P)) {
         free_plan(P);
         return nullptr;
     }
+    // backprojector start table (per view, slab, 8-row group), built once on the device
+    P->SJ = std::max(P->in.XN, P->in.YN);
+    P->SR = (P->SJ + 7) / 8 + 1;
+    const size_t nst = (size_t)P->in.PN * P->SJ * P->SR;
+    if (hip_check(hipMalloc((void**)&P->d_start, nst * sizeof(unsigned short)), "hipMalloc(start table)") ||
+        hip_check(launch_back_start(P->dev(), P->d_rec, P->d_start, P->SJ, P->SR, nullptr), "start table") ||
+        hip_check(hipDeviceSynchronize(), "start table")) {
+        free_plan(P);
+        return nullptr;
+    }
+    P->rec.clear();
+    P->rec.shrink_to_fit();
     return P;
 }
 
@@ -355,8 +382,11 @@ int plan_back(ctlab_plan* P, const float* d_prj, float* d_vol, hipStream_t s)
     const int ny = (int)P->views_y.size(), nx = (int)P->views_x.size();
     const int* vy = P->d_views;
     const int* vx = P->d_views + ny;
-    hipError_t e = P->sat_bits == 32 ? launch_back<float>(g, d_prj, P->d_P, d_vol, vy, ny, vx, nx, s)
-                                     : launch_back<double>(g, d_prj, P->d_P, d_vol, vy, ny, vx, nx, s);
+    hipError_t e = P->sat_bits == 32
+                       ? launch_back<float>(g, d_prj, P->d_P, d_vol, vy, ny, vx, nx, P->d_rec, P->d_start, P->SJ,
+                                            P->SR, s)
+                       : launch_back<double>(g, d_prj, P->d_P, d_vol, vy, ny, vx, nx, P->d_rec, P->d_start, P->SJ,
+                                             P->SR, s);
     if (hip_check(e, "backprojector launch")) return t_err;
     return CTLAB_OK;
 }
@@ -459,6 +489,75 @@ int ctlab_back(ctlab_plan* plan, const float* d_prj, float* d_vol, void* stream)
 {
     clear_error();
     return plan_back(plan, d_prj, d_vol, (hipStream_t)stream);
+}
+
+int ctlab_plan_set_tiles(ctlab_plan* plan, int nu, int nm)
+{
+    clear_error();
+    auto ok = [](int t) { return t == 8 || t == 16 || t == 32; };
+    if (!plan || !ok(nu) || !ok(nm)) {
+        set_error(CTLAB_ERR_ARG, "tile sizes must be 8, 16 or 32");
+        return CTLAB_ERR_ARG;
+    }
+    plan->tile_nu = nu;
+    plan->tile_nm = nm;
+    return CTLAB_OK;
+}
+
+int ctlab_plan_get_tiles(const ctlab_plan* plan, int* nu, int* nm)
+{
+    if (!plan) return CTLAB_ERR_ARG;
+    if (nu) *nu = plan->tile_nu;
+    if (nm) *nm = plan->tile_nm;
+    return CTLAB_OK;
+}
+
+// Times every tile variant of both operators on the caller's buffers (each
+// twice, the faster run counts) and keeps the fastest; d_prj and d_vol are
+// overwritten.  ms_out (optional, 6 floats): proj 8/16/32, back 8/16/32.
+int ctlab_plan_autotune(ctlab_plan* plan, float* d_vol, float* d_prj, float* ms_out)
+{
+    clear_error();
+    if (!plan || !d_vol || !d_prj) {
+        set_error(CTLAB_ERR_ARG, "NULL plan or buffer");
+        return CTLAB_ERR_ARG;
+    }
+    DeviceGuard dg(plan->device);
+    hipStream_t s = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (hip_check(hipStreamCreateWithFlags(&s, hipStreamNonBlocking), "stream") ||
+        hip_check(hipEventCreate(&e0), "event") || hip_check(hipEventCreate(&e1), "event")) {
+        if (s) hipStreamDestroy(s);
+        if (e0) hipEventDestroy(e0);
+        return ctlab_last_error();
+    }
+    const int tiles[3] = {8, 16, 32};
+    float best[2] = {1e30f, 1e30f};
+    int pick[2] = {plan->tile_nu, plan->tile_nm};
+    int rc = CTLAB_OK;
+    for (int op = 0; op < 2 && rc == CTLAB_OK; ++op) {
+        for (int k = 0; k < 3 && rc == CTLAB_OK; ++k) {
+            if (op == 0) plan->tile_nu = tiles[k]; else plan->tile_nm = tiles[k];
+            float tmin = 1e30f;
+            for (int rep = 0; rep < 2 && rc == CTLAB_OK; ++rep) {
+                hipEventRecord(e0, s);
+                rc = op == 0 ? plan_proj(plan, d_vol, d_prj, s) : plan_back(plan, d_prj, d_vol, s);
+                hipEventRecord(e1, s);
+                if (rc == CTLAB_OK && hip_check(hipEventSynchronize(e1), "autotune")) rc = ctlab_last_error();
+                float ms = 0.0f;
+                hipEventElapsedTime(&ms, e0, e1);
+                if (ms < tmin) tmin = ms;
+            }
+            if (ms_out) ms_out[op * 3 + k] = tmin;
+            if (tmin < best[op]) { best[op] = tmin; pick[op] = tiles[k]; }
+        }
+    }
+    plan->tile_nu = pick[0];
+    plan->tile_nm = pick[1];
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    hipStreamDestroy(s);
+    return rc;
 }
 
 int ctlab_plan_views(const ctlab_plan* plan, int* n_yslab, int* n_xslab)

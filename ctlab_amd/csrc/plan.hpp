@@ -24,7 +24,10 @@ template <typename T>
 hipError_t launch_proj(const DevGeom& g, const float* d_vol, void* d_cz, float* d_prj, hipStream_t s);
 template <typename T>
 hipError_t launch_back(const DevGeom& g, const float* d_prj, void* d_P, float* d_vol,
-                       const int* d_views_y, int ny, const int* d_views_x, int nx, hipStream_t s);
+                       const int* d_views_y, int ny, const int* d_views_x, int nx, const void* d_rec,
+                       const unsigned short* d_start, int SJ, int SR, hipStream_t s);
+hipError_t launch_back_start(const DevGeom& g, const void* d_rec, unsigned short* d_start, int SJ, int SR,
+                             hipStream_t s);
 
 }  // namespace ctlab
 
@@ -42,10 +45,15 @@ struct ctlab_plan {
     double *d_zhat = nullptr, *d_ez = nullptr, *d_rowfac = nullptr;
     unsigned char* d_mask = nullptr;
     int* d_views = nullptr;
+    std::vector<double> rec;            // PN*DNU*4: walk-order column records (klo, khi, 1/|dk|, esdz)
+    double* d_rec = nullptr;
+    unsigned short* d_start = nullptr;  // PN*SJ*SR first walk-order column per (view, slab, 8 rows)
+    int SJ = 0, SR = 0;
     void* d_cz = nullptr;   // z-prefix integral image (nvox elements)
     void* d_P = nullptr;    // projection v-prefix (nprj elements)
     int sat_bits = 64;
     size_t ws_elem = 0;
+    int tile_nu = 16, tile_nm = 16;
 
     ctlab::DevGeom dev() const
     {
@@ -57,6 +65,7 @@ struct ctlab_plan {
         g.zhat = d_zhat; g.ez = d_ez; g.rowfac = d_rowfac;
         g.mask = in.mask.empty() ? nullptr : d_mask;
         g.zhat0 = zhat0; g.dzh = dzh;
+        g.tile_nu = tile_nu; g.tile_nm = tile_nm;
         return g;
     }
 };

@@ -104,6 +104,25 @@ class Plan:
         rc = _lib.lib().ctlab_plan_set_sat_bits(self.h, bits)
         _lib.check(rc)
 
+    def set_tiles(self, nu, nm):
+        _lib.check(_lib.lib().ctlab_plan_set_tiles(self.h, nu, nm))
+
+    def tiles(self):
+        a, b = C.c_int(), C.c_int()
+        _lib.lib().ctlab_plan_get_tiles(self.h, C.byref(a), C.byref(b))
+        return a.value, b.value
+
+    def autotune(self, vol, prj):
+        """Times the tile variants on (vol, prj) -- both overwritten -- and keeps
+        the fastest; returns {'proj': {8: ms, 16: ms, 32: ms}, 'back': {...}}."""
+        ms = np.zeros(6, dtype=np.float32)
+        self._check_tensor(vol, self.g.nvox(), "volume")
+        self._check_tensor(prj, self.g.nprj(), "projection")
+        rc = _lib.lib().ctlab_plan_autotune(self.h, C.c_void_p(vol.data_ptr()), C.c_void_p(prj.data_ptr()),
+                                            ms.ctypes.data_as(C.POINTER(C.c_float)))
+        _lib.check(rc)
+        return {"proj": dict(zip((8, 16, 32), ms[:3].tolist())), "back": dict(zip((8, 16, 32), ms[3:].tolist()))}
+
     def views(self):
         a, b = C.c_int(), C.c_int()
         _lib.lib().ctlab_plan_views(self.h, C.byref(a), C.byref(b))

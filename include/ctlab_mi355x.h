@@ -84,6 +84,14 @@ int ctlab_plan_set_sat_bits(ctlab_plan* plan, int bits);
 /* d_vol, d_prj: device pointers in the layouts above; stream: hipStream_t or NULL */
 int ctlab_proj(ctlab_plan* plan, const float* d_vol, float* d_prj, void* stream);
 int ctlab_back(ctlab_plan* plan, const float* d_prj, float* d_vol, void* stream);
+/* tile shapes: detector columns per projector wavefront (nu) and voxel rows
+ * per backprojector wavefront (nm), each 8, 16 or 32 (default 16 / 16) */
+int ctlab_plan_set_tiles(ctlab_plan* plan, int nu, int nm);
+int ctlab_plan_get_tiles(const ctlab_plan* plan, int* nu, int* nm);
+/* autotuner: times every tile variant of both operators on these device
+ * buffers (contents overwritten) and keeps the fastest; ms_out (6 floats or
+ * NULL) receives proj 8/16/32 then back 8/16/32 milliseconds */
+int ctlab_plan_autotune(ctlab_plan* plan, float* d_vol, float* d_prj, float* ms_out);
 /* geometry queries */
 int ctlab_plan_views(const ctlab_plan* plan, int* n_yslab, int* n_xslab);
 long long ctlab_plan_workspace_bytes(const ctlab_plan* plan);

@@ -68,6 +68,26 @@ def test_back_parity_cone(cuda, arc):
     assert e32 <= TOL32
 
 
+@pytest.mark.parametrize("nu,nm", [(8, 32), (32, 8), (16, 16)])
+def test_tile_variants(cuda, nu, nm):
+    import torch
+    g = _cone(XN=40, YN=44, ZN=12, PN=24, DNU=75, DNV=12, ang0=0.3)
+    rng = np.random.default_rng(21)
+    vol = rng.random(g.vol_shape).astype(np.float32)
+    prj = rng.random(g.nprj()).astype(np.float32)
+    pl = dd.Plan(g)
+    pl.set_tiles(nu, nm)
+    assert pl.tiles() == (nu, nm)
+    p = pl.proj(torch.from_numpy(vol).cuda())
+    b = pl.back(torch.from_numpy(prj).cuda())
+    torch.cuda.synchronize()
+    assert rel_l2(p.cpu().numpy(), O.proj(g, vol)) <= TOL64
+    assert rel_l2(b.cpu().numpy(), O.back(g, prj)) <= TOL64
+    ms = pl.autotune(torch.from_numpy(vol).cuda(), torch.empty(g.prj_shape, device="cuda"))
+    assert set(ms["proj"]) == {8, 16, 32} and min(ms["back"].values()) > 0
+    assert pl.tiles()[0] in (8, 16, 32)
+
+
 def test_random_volume_and_mask(cuda):
     rng = np.random.default_rng(13)
     mask = (rng.random(40 * 36) > 0.1).astype(np.uint8)
