@@ -25,6 +25,14 @@ def declare(L):
         "ctlab_soft_threshold": ([_fp, f, ll, _vp], i),
         "ctlab_cg_beta": ([_fp, _fp, _fp, i, ll, _dp, _vp], i),
         "ctlab_tv_prox": ([_fp, _fp, f, i, i, i, i, _fp, _vp], i),
+        "ctlab_cnn_num_params": ([], i),
+        "ctlab_cnn_create": ([i, i, i, i, i], _vp),
+        "ctlab_cnn_destroy": ([_vp], None),
+        "ctlab_cnn_set_params": ([_vp, C.POINTER(C.c_float)], i),
+        "ctlab_cnn_get_params": ([_vp, C.POINTER(C.c_float)], i),
+        "ctlab_cnn_forward": ([_vp, _fp, _fp, _vp], i),
+        "ctlab_cnn_loss_grad": ([_vp, _fp, _fp, _fp, C.POINTER(C.c_float), _vp], d),
+        "ctlab_cnn_adam_step": ([_vp, f, f, f, f, _vp], i),
     }
     for name, (args, res) in sig.items():
         if hasattr(L, name):

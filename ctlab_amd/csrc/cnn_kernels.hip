@@ -112,7 +112,7 @@ k_conv32(const __bf16* __restrict__ in, const __bf16* __restrict__ w, const floa
             }
             if (out_cm) {
                 const long v0 = vbase + b * 16 + 4 * kq;
-                if (v0 + 3 < nvox) {
+                if (v0 + 3 < nvox && (nvox & 3) == 0) {   // 8-byte aligned vector store
                     typedef __bf16 bf16x4 __attribute__((ext_vector_type(4)));
                     *(bf16x4*)(out_cm + (long)co * nvox + v0) = (bf16x4){o[0], o[1], o[2], o[3]};
                 } else {
