@@ -82,6 +82,8 @@ def main():
     ap.add_argument("--cpu-views", type=int, default=16)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-tune", action="store_true")
+    ap.add_argument("--calib-copy-mb", type=int, default=0,
+                    help="first run a plain device-to-device copy of this many MiB (profiler calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the multi-GPU "
@@ -92,6 +94,13 @@ def main():
     from ctlab_amd import dd
 
     dev = torch.device("cuda:0")
+    if args.calib_copy_mb:
+        src = torch.ones(args.calib_copy_mb * (1 << 18), dtype=torch.float32, device=dev)
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
     g = cfg2_geometry()
     vol_np = cfg2_phantom(g)
     plan = dd.Plan(g, sat_bits=args.sat_bits)
