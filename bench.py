@@ -73,6 +73,74 @@ def cpu_baseline(g, vol, prj_gpu, back_fn, nviews):
     }, {"proj_rel_l2_vs_fp64": e_p, "back_rel_l2_vs_fp64": e_b, "views_checked": nviews}
 
 
+def learned_bench(args):
+    """cfg5: train the learned prox on projector-generated pairs, then learned
+    safeguarded FISTA vs plain FISTA-TV on a 512x512x64 phantom (120 views, I0 1e4)."""
+    import torch
+    from ctlab_amd import learned as LE
+    from ctlab_amd import dd
+    from ctlab_amd import recon as RC
+    dev = torch.device("cuda:0")
+    log = lambda m: print(m, file=sys.stderr, flush=True)
+    tshape = (256, 256, 32)
+    xs, ts, Lt, t_gen = LE.generate_pairs(args.samples, tshape, 120, 1e4, 1000, dev, log=log)
+    params, losses, t_train, steps = LE.train(xs, ts, tshape, args.epochs, lr=1e-3, log=log)
+    del xs, ts
+    np.save(os.path.join(ROOT, "gpurun_out", "learned_prox_params.npy"), params) if os.path.isdir(
+        os.path.join(ROOT, "gpurun_out")) else None
+    # plain FISTA-TV lambda chosen on a held-out validation phantom (seed 999) at the training size
+    gv = LE.cone_geometry(tshape[1], tshape[0], tshape[2], 120)
+    pv = dd.Plan(gv)
+    rv = RC.Recon(pv, 1)
+    from ctlab_amd import phantom as PH
+    ph_v = LE.voxelize_torch(PH.random_ellipsoids(999, 20, gv.XN, gv.YN, gv.ZN, gv.dx, gv.dz), gv.vol_shape,
+                             gv.dx, gv.dz, dev)
+    rv.set_data(LE.noisy_projection(pv, ph_v, 1e4, 999))
+    zv = LE.covered_slices(gv)
+    sweep = {}
+    for lam in (1e-3, 1e-2, 3e-2, 1e-1, 3e-1):
+        _, hv = LE.fista_tv_run(rv, Lt, gv.vol_shape, args.iters, lam, 10, ph_v, zv, dev)
+        sweep[lam] = max(h["psnr"] for h in hv)
+    lam = max(sweep, key=sweep.get)
+    # evaluation phantom, 512x512x64, seed 7
+    ge = LE.cone_geometry(512, 512, 64, 120)
+    pe = dd.Plan(ge)
+    re_ = RC.Recon(pe, 1)
+    ph = LE.voxelize_torch(PH.random_ellipsoids(7, 20, 512, 512, 64, ge.dx, ge.dz), ge.vol_shape, ge.dx, ge.dz, dev)
+    re_.set_data(LE.noisy_projection(pe, ph, 1e4, 7))
+    Le = re_.lipschitz(30) * 1.01
+    ze = LE.covered_slices(ge)
+    _, h_plain = LE.fista_tv_run(re_, Le, ge.vol_shape, args.iters, lam, 10, ph, ze, dev)
+    net = LE.ScaledCNN(ge.vol_shape, params)
+    _, h_learn = LE.fista_tv_run(re_, Le, ge.vol_shape, args.iters, lam, 10, ph, ze, dev, learned=net)
+    target = 40.0
+    best_plain = max(h["psnr"] for h in h_plain)
+    tp = LE.time_to_psnr(h_plain, target)
+    tl = LE.time_to_psnr(h_learn, target)
+    # if plain never reaches 40 dB, compare at the best PSNR plain FISTA-TV reaches
+    ref_db = target if tp[0] is not None else best_plain
+    tp2, tl2 = LE.time_to_psnr(h_plain, ref_db), LE.time_to_psnr(h_learn, ref_db)
+    rate, t_fwd = LE.cnn_forward_rate(ge.vol_shape, params, dev)
+    res = {
+        "metric": METRIC, "mode": "cfg5 learned safeguarded FISTA vs plain FISTA-TV",
+        "train": {"pairs": args.samples, "shape": list(tshape), "views": 120, "I0": 1e4, "epochs": args.epochs,
+                  "steps": steps, "gen_s": t_gen, "train_s": t_train, "loss_per_epoch": losses},
+        "tv_lambda_sweep_val_psnr": {str(k): v for k, v in sweep.items()}, "tv_lambda": lam,
+        "eval": {"shape": [512, 512, 64], "views": 120, "I0": 1e4, "iters": args.iters, "psnr_slices": [ze.start, ze.stop]},
+        "plain": {"best_psnr": best_plain, "final_psnr": h_plain[-1]["psnr"], "t_total_s": h_plain[-1]["t"],
+                  "t_to_ref_s": tp2[0], "iters_to_ref": tp2[1], "fidelity_at_ref": tp2[2]},
+        "learned": {"best_psnr": max(h["psnr"] for h in h_learn), "final_psnr": h_learn[-1]["psnr"],
+                    "t_total_s": h_learn[-1]["t"], "t_to_ref_s": tl2[0], "iters_to_ref": tl2[1],
+                    "fidelity_at_ref": tl2[2], "accepted_steps": sum(h["learned"] for h in h_learn)},
+        "ref_psnr_db": ref_db, "reached_40dB": {"plain": tp[0] is not None, "learned": tl[0] is not None},
+        "speedup_to_ref": (tp2[0] / tl2[0]) if (tp2[0] and tl2[0]) else None,
+        "cnn_forward": {"shape": [512, 512, 64], "s": t_fwd, "tflops": rate / 1e12,
+                        "mfma_util_vs_bf16_dense_peak": rate / 2.5e15},
+        "history": {"plain": h_plain, "learned": h_learn},
+    }
+    print(json.dumps(res))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -82,6 +150,10 @@ def main():
     ap.add_argument("--cpu-views", type=int, default=16)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-tune", action="store_true")
+    ap.add_argument("--learned", action="store_true", help="cfg5: learned FISTA vs plain FISTA-TV")
+    ap.add_argument("--samples", type=int, default=2000)
+    ap.add_argument("--epochs", type=int, default=3)
+    ap.add_argument("--iters", type=int, default=60)
     ap.add_argument("--calib-copy-mb", type=int, default=0,
                     help="first run a plain device-to-device copy of this many MiB (profiler calibration)")
     args = ap.parse_args()
@@ -89,6 +161,9 @@ def main():
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the multi-GPU "
               "view/slab split is design-only in this build.", file=sys.stderr)
         sys.exit(2)
+
+    if args.learned:
+        return learned_bench(args)
 
     import torch
     from ctlab_amd import dd

@@ -23,8 +23,16 @@ def declare(L):
         "ctlab_ossart_update": ([_fp, _fp, _fp, _fp, f, ll, _vp], i),
         "ctlab_fista_momentum": ([_fp, _fp, f, f, f, f, ll, _vp], i),
         "ctlab_soft_threshold": ([_fp, f, ll, _vp], i),
+        "ctlab_axpby": ([f, _fp, f, _fp, ll, _vp], i),
         "ctlab_cg_beta": ([_fp, _fp, _fp, i, ll, _dp, _vp], i),
         "ctlab_tv_prox": ([_fp, _fp, f, i, i, i, i, _fp, _vp], i),
+        "ctlab_batch_create": ([C.POINTER(_vp), i], _vp),
+        "ctlab_batch_destroy": ([_vp], None),
+        "ctlab_batch_size": ([_vp], i),
+        "ctlab_batch_set_tiles": ([_vp, i, i], i),
+        "ctlab_batch_get_tiles": ([_vp, C.POINTER(i), C.POINTER(i)], i),
+        "ctlab_batch_os_sart": ([_vp, C.POINTER(_vp), i, f, i, i, _vp], i),
+        "ctlab_batch_autotune": ([_vp, C.POINTER(_vp), f, C.POINTER(f)], i),
         "ctlab_cnn_num_params": ([], i),
         "ctlab_cnn_create": ([i, i, i, i, i], _vp),
         "ctlab_cnn_destroy": ([_vp], None),

@@ -13,7 +13,22 @@
 #include <hip/hip_runtime.h>
 #include <cstdint>
 
+// Device pointers in the structs below address global memory.  The device pass
+// says so in their types: a kernel that reads such a pointer out of a
+// descriptor array (not a kernel argument) would otherwise get flat loads,
+// which also count on the LDS wait counter and would serialise the walk
+// loops' LDS reads behind their prefetched memory loads.
+#if defined(__HIP_DEVICE_COMPILE__)
+#define CTLAB_GMEM __attribute__((address_space(1)))
+#else
+#define CTLAB_GMEM
+#endif
+
 namespace ctlab {
+
+// generic -> global pointer (explicit address-space cast in the device pass)
+template <typename T>
+__host__ __device__ inline CTLAB_GMEM T* gmem(T* p) { return (CTLAB_GMEM T*)p; }
 
 // Per-view constants, computed on the host in FP64 (plan build).
 struct ViewParams {
@@ -32,20 +47,20 @@ struct DevGeom {
     int XN, YN, ZN, DNU, DNV, PN;
     double dx, dz;
     int cone;                 // 1: cone beam, 0: multi-slice fan beam
-    const ViewParams* vp;     // PN
-    const double* kap;        // PN*(DNU+1): d(row)/d(slab) slope of each u-boundary ray
-    const double* tau;        // PN*DNU: 1 / (cell-centre slab coord - source slab coord)
-    const double* colfac;     // PN*DNU: per-column output factor
-    const double* exy2;       // PN*DNU: squared in-plane source->cell-centre distance
-    const double* invdk;      // PN*DNU: 1/|kap[u+1]-kap[u]|
-    const double* esdz;       // PN*DNU: (cell-centre slab coord - source slab coord) / dzh
-    const double* zhat;       // DNV+1: row boundaries, (z - z0)/dz (cone) or v (fan)
-    const double* ez;         // DNV: row centre z - z0 (0 for fan beam)
-    const double* rowfac;     // DNV: 1/(zhat[v+1]-zhat[v]) (cone) or 1 (fan)
-    const unsigned char* mask;// XN*YN or nullptr
+    const CTLAB_GMEM ViewParams* vp;     // PN
+    const CTLAB_GMEM double* kap;        // PN*(DNU+1): d(row)/d(slab) slope of each u-boundary ray
+    const CTLAB_GMEM double* tau;        // PN*DNU: 1 / (cell-centre slab coord - source slab coord)
+    const CTLAB_GMEM double* colfac;     // PN*DNU: per-column output factor
+    const CTLAB_GMEM double* exy2;       // PN*DNU: squared in-plane source->cell-centre distance
+    const CTLAB_GMEM double* invdk;      // PN*DNU: 1/|kap[u+1]-kap[u]|
+    const CTLAB_GMEM double* esdz;       // PN*DNU: (cell-centre slab coord - source slab coord) / dzh
+    const CTLAB_GMEM double* zhat;       // DNV+1: row boundaries, (z - z0)/dz (cone) or v (fan)
+    const CTLAB_GMEM double* ez;         // DNV: row centre z - z0 (0 for fan beam)
+    const CTLAB_GMEM double* rowfac;     // DNV: 1/(zhat[v+1]-zhat[v]) (cone) or 1 (fan)
+    const CTLAB_GMEM unsigned char* mask;// XN*YN or nullptr
     double zhat0, dzh;        // uniform row model used by the backprojector
-    int tile_nu;              // projector: detector columns per wavefront (8, 16, 32)
-    int tile_nm;              // backprojector: voxel rows per wavefront (8, 16, 32)
+    int tile_nu;              // projector: detector columns per wavefront (8, 12, 16)
+    int tile_nm;              // backprojector: voxel rows per wavefront (8, 12, 16)
 };
 
 }  // namespace ctlab

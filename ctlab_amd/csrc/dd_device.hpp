@@ -1,0 +1,315 @@
+// Per-wavefront bodies of the branchless DD kernels, shared by the per-plan
+// kernels (dd_kernels.hip) and the batched persistent kernels (batch.hip).
+#pragma once
+#include "dd_common.hpp"
+
+namespace ctlab {
+
+// Intra-wavefront hand-off through LDS (every lane writes its prefix entry,
+// then reads two entries written by other lanes): the LDS executes one
+// wavefront's instructions in issue order, and the compiler keeps the store /
+// load order because the indices may alias, so no fence is needed (a fence
+// would also wait for the prefetched global loads still in flight).
+
+// uniform read of a double held by lane l
+__device__ __forceinline__ double readlane_d(double v, int l)
+{
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_readlane((int)(b & 0xffffffffLL), l);
+    const int hi = __builtin_amdgcn_readlane((int)(b >> 32), l);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+
+// piecewise-linear read of a prefix table G[0..n] (G[0] = 0) at b in [0, n]
+template <typename T>
+__device__ __forceinline__ T lds_interp(const T* G, double b, int n)
+{
+    b = fmin(fmax(b, 0.0), (double)n);
+    const int i = min((int)b, n - 1);
+    const T fr = (T)(b - (double)i);
+    const T g0 = G[i];
+    const T g1 = G[i + 1];
+    return g0 + fr * (g1 - g0);
+}
+
+// ---------------------------------------------------------------------------
+// z-prefix of every masked voxel column: cz[col*ZN + k] = sum_{k'<=k} f*mask
+// ---------------------------------------------------------------------------
+// inclusive prefix over the 64 lanes of a wavefront (lane l gets sum of lanes 0..l)
+template <typename T>
+__device__ __forceinline__ T wave_inclusive_scan(T v, int lane)
+{
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const T n = __shfl_up(v, o, 64);
+        if (lane >= o) v += n;
+    }
+    return v;
+}
+
+// inclusive z-prefix of one masked voxel column (lane = z; ZN <= 64)
+template <typename T>
+__device__ __forceinline__ void zprefix_wave(const CTLAB_GMEM float* __restrict__ vol, CTLAB_GMEM T* __restrict__ cz,
+                                             const CTLAB_GMEM unsigned char* __restrict__ mask, long col, int ZN, int lane)
+{
+    const T mk = (mask == nullptr || mask[col]) ? T(1) : T(0);
+    const T v = lane < ZN ? (T)vol[col * ZN + lane] * mk : T(0);
+    const T r = wave_inclusive_scan(v, lane);
+    if (lane < ZN) cz[col * ZN + lane] = r;
+}
+
+// Projector walk of one (view p, tile of NU detector columns) over all slabs.
+// Lane = z for the SAT rows, lane = v for outputs; G: 65+ doubles of LDS.
+template <typename T, int NU>
+__device__ __forceinline__ void proj_walk_wave(const DevGeom& g, const CTLAB_GMEM T* __restrict__ cz,
+                                               CTLAB_GMEM float* __restrict__ prj,
+                                               int p, int tile, int lane, T* G)
+{
+    if (lane == 0) G[0] = T(0);
+    const ViewParams vp = g.vp[p];
+    const int u0 = tile * NU;
+    const int nu = min(NU, g.DNU - u0);
+    // this tile's boundary slopes (lanes 0..nu) and magnifications (lanes 0..nu-1)
+    // are the same for every slab: load once, read per column with v_readlane
+    const double kap_l = g.kap[(long)p * (g.DNU + 1) + u0 + min(lane, nu)];
+    const double tau_l = g.tau[(long)p * g.DNU + u0 + min(lane, nu - 1)];
+    const int ZN = g.ZN, NR = vp.NR, DNV = g.DNV;
+    const int zl = min(lane, ZN - 1);
+    const int vl = min(lane, DNV - 1);
+    const double zlo = g.zhat[vl], zhi = g.zhat[vl + 1];
+    const double inv_dx = 1.0 / g.dx;
+    const long slab_off = (long)vp.slab_stride * ZN;
+    const long row_off = (long)vp.row_stride * ZN;
+    const long last = (long)(NR - 1) * row_off;
+
+    T acc[NU];
+#pragma unroll
+    for (int q = 0; q < NU; ++q) acc[q] = T(0);
+
+    for (int j = 0; j < vp.NS; ++j) {
+        const double sj = vp.s0 + j * g.dx;
+        const double rs = sj * inv_dx;
+        const double afirst = vp.rbase + rs * readlane_d(kap_l, vp.asc ? 0 : nu);
+        const double alast = vp.rbase + rs * readlane_d(kap_l, vp.asc ? nu : 0);
+        if (alast <= 0.0 || afirst >= (double)NR) continue;
+        const T qj = (T)(g.cone ? 1.0 / (sj * sj) : 1.0 / fabs(sj));
+        const CTLAB_GMEM T* __restrict__ base = cz + j * slab_off + zl;
+        double ap = fmin(fmax(afirst, 0.0), (double)NR);
+        int m = min((int)ap, NR - 1);
+        double mf = (double)m;
+        long off = m * row_off;
+        // rows m, m+1, m+2 in flight (indices clamped to the last row)
+        T c0 = base[off];
+        T c1 = base[min(off + row_off, last)];
+        T c2 = base[min(off + 2 * row_off, last)];
+        T S = T(0);
+        T Eprev = (T)(ap - mf) * c0;
+        for (int q = 1; q <= nu; ++q) {
+            double aq = vp.rbase + rs * readlane_d(kap_l, vp.asc ? q : nu - q);
+            aq = fmin(fmax(aq, 0.0), (double)NR);
+            while (m < NR - 1 && aq >= mf + 1.0) {
+                S += c0;
+                ++m;
+                mf += 1.0;
+                off += row_off;
+                c0 = c1;
+                c1 = c2;
+                c2 = base[min(off + 2 * row_off, last)];
+            }
+            const T E = S + (T)(aq - mf) * c0;
+            const T H = E - Eprev;
+            Eprev = E;
+            if (aq > ap) {
+                const double t = g.cone ? sj * readlane_d(tau_l, vp.asc ? q - 1 : nu - q) : 1.0;
+                G[lane + 1] = H;
+                const T Dhi = lds_interp(G, vp.beta0 + t * zhi, ZN);
+                const T Dlo = lds_interp(G, vp.beta0 + t * zlo, ZN);
+                acc[q - 1] += (Dhi - Dlo) * qj;
+            }
+            ap = aq;
+        }
+    }
+
+    if (lane < DNV) {
+        const double ezv = g.ez[lane];
+        const double rf = g.rowfac[lane];
+        CTLAB_GMEM float* __restrict__ out = prj + (long)p * g.DNU * DNV + lane;
+#pragma unroll
+        for (int q = 0; q < NU; ++q) {
+            if (q >= nu) break;
+            const int u = vp.asc ? u0 + q : u0 + nu - 1 - q;
+            const long cu = (long)p * g.DNU + u;
+            const double val = (double)acc[q] * g.colfac[cu] * sqrt(g.exy2[cu] + ezv * ezv) * rf;
+            out[(long)u * DNV] = (float)val;
+        }
+    }
+}
+
+
+// inclusive v-prefix of y*L for one (view, column) row idx = p*DNU + u (lane = v)
+// With real / rowSum (SART weighting, src/SARTWeighting.h:22-37, eps = 1e-7f),
+// y is first replaced by (real - y) / rowSum, 0 where rowSum <= 1e-7.
+template <typename T>
+__device__ __forceinline__ void prjprefix_wave(const DevGeom& g, const CTLAB_GMEM float* __restrict__ prj,
+                                               CTLAB_GMEM T* __restrict__ P, long idx, int lane,
+                                               const CTLAB_GMEM float* __restrict__ real = nullptr,
+                                               const CTLAB_GMEM float* __restrict__ rowSum = nullptr)
+{
+    const int DNV = g.DNV;
+    T v = T(0);
+    if (lane < DNV) {
+        const double ezv = g.ez[lane];
+        const double L = g.dx * sqrt(g.exy2[idx] + ezv * ezv) * fabs(g.tau[idx]);
+        float y = prj[idx * DNV + lane];
+        if (real) {
+            const float w = rowSum[idx * DNV + lane];
+            y = w > 1.0e-7f ? (real[idx * DNV + lane] - y) / w : 0.0f;
+        }
+        v = (T)((double)y * L);
+    }
+    const T r = wave_inclusive_scan(v, lane);
+    if (lane < DNV) P[idx * DNV + lane] = r;
+}
+
+// Adjoint z-part for one detector column: sum_v y*L * (fraction of cell v
+// covered by the voxel z-interval [lane, lane+1] mapped onto the detector rows),
+// read from the column's v-prefix Pu (lane v holds prefix v+1) through LDS.
+// esdz = (cell-centre slab coordinate - source) / row pitch, so that the
+// voxel boundary b maps to the fractional row (b - beta0) * esdz / sj - zhat0/dzh.
+template <typename T>
+__device__ __forceinline__ T column_delta(const DevGeom& g, T* G, T Pu, double esdz, double inv_sj,
+                                          double beta0, int lane)
+{
+    const int DNV = g.DNV;
+    G[lane + 1] = Pu;
+    double nlo, nhi;
+    T sgn = T(1);
+    if (g.cone) {
+        const double inv = inv_sj * esdz;   // 1 / (t * dzh)
+        const double off = -beta0 * inv - g.zhat0 / g.dzh;
+        nlo = (double)lane * inv + off;
+        nhi = nlo + inv;
+        if (g.dzh < 0.0) sgn = T(-1);
+    } else {
+        nlo = (double)lane;
+        nhi = (double)(lane + 1);
+    }
+    return (lds_interp(G, nhi, DNV) - lds_interp(G, nlo, DNV)) * sgn;
+}
+
+// Column record in walk order (increasing row coordinate a): slopes of the
+// column's lower and upper boundary rays, 1/|slope step|, and esdz.
+struct ColRec { double klo, khi, invdk, esdz; };
+
+// Backprojector walk of one (slab j, tile rt of NM rows) over the views of one
+// slab orientation; lane = z.  acc[i] (zeroed here) receives voxel row
+// rt*NM + i.  See dd_back_walk for the structure.
+template <typename T, int NM>
+__device__ __forceinline__ void back_walk_accum(const DevGeom& g, const CTLAB_GMEM T* __restrict__ P,
+                                                const CTLAB_GMEM int* __restrict__ views, int nviews, int NR, int j,
+                                                int rt, const CTLAB_GMEM ColRec* __restrict__ rec,
+                                                const CTLAB_GMEM unsigned short* __restrict__ start,
+                                                int SJ, int SR, int lane, T* G, T (&acc)[NM])
+{
+    if (lane == 0) G[0] = T(0);
+    const int r0 = rt * NM;
+    const int nm = min(NM, NR - r0);
+    const int ZN = g.ZN, DNU = g.DNU, DNV = g.DNV;
+    const int vl = min(lane, DNV - 1);
+    const double inv_dx = 1.0 / g.dx;
+    const long srow = (long)j * SR + (r0 >> 3);
+
+#pragma unroll
+    for (int i = 0; i < NM; ++i) acc[i] = T(0);
+
+    int pn = nviews > 0 ? views[0] : 0;
+    int stn = nviews > 0 ? start[(long)pn * SJ * SR + srow] : 0xFFFF;
+    for (int iv = 0; iv < nviews; ++iv) {
+        const int p = __builtin_amdgcn_readfirstlane(pn);
+        const int st = __builtin_amdgcn_readfirstlane(stn);
+        if (iv + 1 < nviews) {
+            pn = views[iv + 1];
+            stn = start[(long)pn * SJ * SR + srow];
+        }
+        if (st == 0xFFFF) continue;
+        const ViewParams vp = g.vp[p];
+        const double sj = vp.s0 + j * g.dx;
+        const double rs = sj * inv_dx;
+        const double inv_sj = 1.0 / sj;
+        const double inv_rs = fabs(g.dx * inv_sj);
+        const int asc = vp.asc;
+        const CTLAB_GMEM ColRec* __restrict__ R = rec + (long)p * DNU;
+        const CTLAB_GMEM T* __restrict__ Pv = P + (long)p * DNU * DNV + vl;
+        int c = st;
+        ColRec r = R[c];
+        double alo = vp.rbase + rs * r.klo;
+        if (alo >= (double)(r0 + nm)) continue;
+        double ahi = vp.rbase + rs * r.khi;
+        double iw = inv_rs * r.invdk;
+        const int c1 = min(c + 1, DNU - 1);
+        T Pr0 = Pv[(long)(asc ? c : DNU - 1 - c) * DNV];
+        ColRec rn = R[c1];
+        T Pr1 = Pv[(long)(asc ? c1 : DNU - 1 - c1) * DNV];
+        T D = column_delta(g, G, Pr0, r.esdz, inv_sj, vp.beta0, lane);
+        T Tsum = T(0), Wprev = T(0);
+#pragma unroll
+        for (int i = 0; i <= NM; ++i) {
+            if (i > nm) break;
+            const double mp = (double)(r0 + i);
+            while (ahi <= mp && c < DNU - 1) {
+                Tsum += D;
+                ++c;
+                alo = ahi;
+                ahi = vp.rbase + rs * rn.khi;
+                iw = inv_rs * rn.invdk;
+                D = column_delta(g, G, Pr1, rn.esdz, inv_sj, vp.beta0, lane);
+                const int cn = min(c + 1, DNU - 1);
+                rn = R[cn];
+                Pr1 = Pv[(long)(asc ? cn : DNU - 1 - cn) * DNV];
+            }
+            const double fr = fmin(fmax((mp - alo) * iw, 0.0), 1.0);
+            const T W = Tsum + (T)fr * D;
+            if (i > 0) acc[i - 1] += W - Wprev;
+            Wprev = W;
+        }
+    }
+}
+
+// Writes one backprojected tile (lane = z): for voxel row i of the tile,
+// store(idx, v) receives the volume index and the masked value.
+template <typename T, int NM, class Store>
+__device__ __forceinline__ void back_store_tile(const DevGeom& g, const T (&acc)[NM], int yslab, int NR, int j,
+                                                int rt, int lane, Store&& store)
+{
+    const int r0 = rt * NM;
+    const int nm = min(NM, NR - r0);
+    if (lane < g.ZN) {
+#pragma unroll
+        for (int i = 0; i < NM; ++i) {
+            if (i >= nm) break;
+            const int m = r0 + i;
+            const long col = yslab ? ((long)j * g.XN + m) : ((long)m * g.XN + j);
+            const float mk = (g.mask == nullptr || g.mask[col]) ? 1.0f : 0.0f;
+            store(col * g.ZN + lane, (float)acc[i] * mk);
+        }
+    }
+}
+
+// Backprojection of one (slab j, row tile rt) over the listed views of one
+// slab orientation; overwrites (accumulate = 0) or adds to vol.
+template <typename T, int NM>
+__device__ __forceinline__ void back_walk_wave(const DevGeom& g, const CTLAB_GMEM T* __restrict__ P,
+                                               CTLAB_GMEM float* __restrict__ vol, const CTLAB_GMEM int* __restrict__ views,
+                                               int nviews, int yslab, int accumulate, int NR, int j, int rt,
+                                               const CTLAB_GMEM ColRec* __restrict__ rec,
+                                               const CTLAB_GMEM unsigned short* __restrict__ start, int SJ, int SR, int lane,
+                                               T* G)
+{
+    T acc[NM];
+    back_walk_accum<T, NM>(g, P, views, nviews, NR, j, rt, rec, start, SJ, SR, lane, G, acc);
+    back_store_tile<T, NM>(g, acc, yslab, NR, j, rt, lane, [&](long idx, float v) {
+        vol[idx] = accumulate ? vol[idx] + v : v;
+    });
+}
+
+}  // namespace ctlab

@@ -244,6 +244,7 @@ void free_plan(ctlab_plan* P)
     int cur = 0;
     hipGetDevice(&cur);
     hipSetDevice(P->device);
+    if (P->ws_shared) P->d_cz = P->d_P = nullptr;
     void* ptrs[] = {P->d_vp, P->d_kap, P->d_tau, P->d_colfac, P->d_exy2, P->d_invdk, P->d_esdz, P->d_zhat, P->d_ez, P->d_rec, P->d_start,
                     P->d_rowfac, P->d_mask, P->d_views, P->d_cz, P->d_P};
     for (void* q : ptrs)
@@ -257,16 +258,26 @@ static int alloc_workspace(ctlab_plan* P)
     const size_t es = P->sat_bits == 32 ? 4 : 8;
     const size_t nvox = (size_t)P->in.XN * P->in.YN * P->in.ZN;
     const size_t nprj = (size_t)P->in.PN * P->in.DNU * P->in.DNV;
-    if (P->d_cz) hipFree(P->d_cz);
-    if (P->d_P) hipFree(P->d_P);
+    if (!P->ws_shared) {
+        if (P->d_cz) hipFree(P->d_cz);
+        if (P->d_P) hipFree(P->d_P);
+    }
     P->d_cz = P->d_P = nullptr;
+    P->ws_shared = false;
     if (hip_check(hipMalloc(&P->d_cz, nvox * es), "hipMalloc(integral image)")) return -1;
     if (hip_check(hipMalloc(&P->d_P, nprj * es), "hipMalloc(projection prefix)")) return -1;
     P->ws_elem = es;
     return 0;
 }
 
-ctlab_plan* finish_plan(ctlab_plan* P, int gpunum)
+static bool can_share(const ctlab_plan* P, const ctlab_plan* W)
+{
+    return W && W->device == P->device && W->sat_bits == P->sat_bits && W->in.XN == P->in.XN &&
+           W->in.YN == P->in.YN && W->in.ZN == P->in.ZN && W->in.DNU == P->in.DNU && W->in.DNV == P->in.DNV &&
+           W->in.PN >= P->in.PN && W->d_cz && W->d_P;
+}
+
+ctlab_plan* finish_plan(ctlab_plan* P, int gpunum, const ctlab_plan* share_ws)
 {
     if (build_tables(P)) { delete P; return nullptr; }
     int ndev = 0;
@@ -284,7 +295,10 @@ ctlab_plan* finish_plan(ctlab_plan* P, int gpunum)
         upload(&P->d_colfac, P->colfac) || upload(&P->d_exy2, P->exy2) || upload(&P->d_invdk, P->invdk) ||
         upload(&P->d_esdz, P->esdz) || upload(&P->d_zhat, P->zhat) ||
         upload(&P->d_ez, P->ez) || upload(&P->d_rowfac, P->rowfac) || upload(&P->d_mask, P->in.mask) ||
-        upload(&P->d_views, views) || upload(&P->d_rec, P->rec) || alloc_workspace(P)) {
+        upload(&P->d_views, views) || upload(&P->d_rec, P->rec) ||
+        (can_share(P, share_ws) ? (P->d_cz = share_ws->d_cz, P->d_P = share_ws->d_P, P->ws_elem = share_ws->ws_elem,
+                                   P->ws_shared = true, 0)
+                                : alloc_workspace(P))) {
         free_plan(P);
         return nullptr;
     }
@@ -494,9 +508,9 @@ int ctlab_back(ctlab_plan* plan, const float* d_prj, float* d_vol, void* stream)
 int ctlab_plan_set_tiles(ctlab_plan* plan, int nu, int nm)
 {
     clear_error();
-    auto ok = [](int t) { return t == 8 || t == 16 || t == 32; };
+    auto ok = [](int t) { return t == 8 || t == 12 || t == 16; };
     if (!plan || !ok(nu) || !ok(nm)) {
-        set_error(CTLAB_ERR_ARG, "tile sizes must be 8, 16 or 32");
+        set_error(CTLAB_ERR_ARG, "tile sizes must be 8, 12 or 16");
         return CTLAB_ERR_ARG;
     }
     plan->tile_nu = nu;
@@ -514,7 +528,7 @@ int ctlab_plan_get_tiles(const ctlab_plan* plan, int* nu, int* nm)
 
 // Times every tile variant of both operators on the caller's buffers (each
 // twice, the faster run counts) and keeps the fastest; d_prj and d_vol are
-// overwritten.  ms_out (optional, 6 floats): proj 8/16/32, back 8/16/32.
+// overwritten.  ms_out (optional, 6 floats): proj 8/12/16, back 8/12/16.
 int ctlab_plan_autotune(ctlab_plan* plan, float* d_vol, float* d_prj, float* ms_out)
 {
     clear_error();
@@ -531,7 +545,7 @@ int ctlab_plan_autotune(ctlab_plan* plan, float* d_vol, float* d_prj, float* ms_
         if (e0) hipEventDestroy(e0);
         return ctlab_last_error();
     }
-    const int tiles[3] = {8, 16, 32};
+    const int tiles[3] = {8, 12, 16};
     float best[2] = {1e30f, 1e30f};
     int pick[2] = {plan->tile_nu, plan->tile_nm};
     int rc = CTLAB_OK;

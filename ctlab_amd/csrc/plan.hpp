@@ -53,6 +53,7 @@ struct ctlab_plan {
     void* d_P = nullptr;    // projection v-prefix (nprj elements)
     int sat_bits = 64;
     size_t ws_elem = 0;
+    bool ws_shared = false; // d_cz / d_P belong to another plan (subset plans of a reconstruction)
     int tile_nu = 16, tile_nm = 16;
 
     ctlab::DevGeom dev() const
@@ -60,10 +61,11 @@ struct ctlab_plan {
         ctlab::DevGeom g;
         g.XN = in.XN; g.YN = in.YN; g.ZN = in.ZN; g.DNU = in.DNU; g.DNV = in.DNV; g.PN = in.PN;
         g.dx = in.dx; g.dz = in.dz; g.cone = in.cone;
-        g.vp = d_vp; g.kap = d_kap; g.tau = d_tau; g.colfac = d_colfac; g.exy2 = d_exy2;
-        g.invdk = d_invdk; g.esdz = d_esdz;
-        g.zhat = d_zhat; g.ez = d_ez; g.rowfac = d_rowfac;
-        g.mask = in.mask.empty() ? nullptr : d_mask;
+        g.vp = ctlab::gmem(d_vp); g.kap = ctlab::gmem(d_kap); g.tau = ctlab::gmem(d_tau);
+        g.colfac = ctlab::gmem(d_colfac); g.exy2 = ctlab::gmem(d_exy2);
+        g.invdk = ctlab::gmem(d_invdk); g.esdz = ctlab::gmem(d_esdz);
+        g.zhat = ctlab::gmem(d_zhat); g.ez = ctlab::gmem(d_ez); g.rowfac = ctlab::gmem(d_rowfac);
+        g.mask = in.mask.empty() ? nullptr : ctlab::gmem(d_mask);
         g.zhat0 = zhat0; g.dzh = dzh;
         g.tile_nu = tile_nu; g.tile_nm = tile_nm;
         return g;
@@ -71,8 +73,10 @@ struct ctlab_plan {
 };
 
 namespace ctlab {
-// builds the device tables of a plan whose `in` is filled; frees P and returns null on error
-ctlab_plan* finish_plan(ctlab_plan* P, int gpunum);
+// builds the device tables of a plan whose `in` is filled; frees P and returns null on error.
+// With share_ws, the plan borrows that plan's integral-image / prefix workspace
+// (same SAT precision, at least as large); the two must then run in sequence.
+ctlab_plan* finish_plan(ctlab_plan* P, int gpunum, const ctlab_plan* share_ws = nullptr);
 void free_plan(ctlab_plan* P);
 int plan_proj(ctlab_plan* P, const float* d_vol, float* d_prj, hipStream_t s);
 int plan_back(ctlab_plan* P, const float* d_prj, float* d_vol, hipStream_t s);

@@ -18,6 +18,7 @@
 
 #include "../../include/ctlab_mi355x.h"
 #include "plan.hpp"
+#include "recon.hpp"
 #include "status.hpp"
 
 namespace ctlab {
@@ -42,34 +43,6 @@ hipError_t rk_tv_prox(const float* b, float* x, float lam, int iters, int XN, in
 
 using namespace ctlab;
 
-struct ctlab_recon {
-    int device = 0;
-    int osNum = 1;
-    long nvox = 0, nprj = 0;
-    int XN = 0, YN = 0, ZN = 0;
-    std::vector<ctlab_plan*> sub;       // subset plans (interleaved views)
-    std::vector<long> sub_off;          // element offset of each subset in the permuted projections
-    std::vector<int> perm;              // permuted position -> original view
-    ctlab_plan* full = nullptr;         // all views, original order (FISTA / CG)
-    float* d_b = nullptr;               // measured projections, original order
-    float* d_bperm = nullptr;           // measured projections, subset order
-    float* d_rowsum = nullptr;          // per subset A_s * 1 (subset order)
-    float* d_colsum = nullptr;          // per subset A_s^T * 1 (osNum volumes)
-    float* d_prj = nullptr;             // projection work buffer (nprj)
-    float* d_vol = nullptr;             // volume work buffers
-    float* d_vol2 = nullptr;
-    float* d_vol3 = nullptr;
-    float* d_vol4 = nullptr;
-    float* d_tv = nullptr;              // 7 * nvox for the FGP prox
-    double* d_part = nullptr;           // reduction partials
-    double* d_red = nullptr;            // reduction results (4 doubles)
-    bool have_data = false;
-    hipGraphExec_t sart_graph = nullptr;
-    hipStream_t graph_stream = nullptr;
-    float graph_lambda = 0.0f;
-    int graph_nonneg = -1;
-    float* graph_x = nullptr;           // volume the captured graph updates in place
-};
 
 namespace {
 
@@ -92,7 +65,7 @@ void destroy(ctlab_recon* R)
     delete R;
 }
 
-ctlab_plan* subset_plan(const ctlab_plan* P, const std::vector<int>& views)
+ctlab_plan* subset_plan(const ctlab_plan* P, const std::vector<int>& views, const ctlab_plan* share_ws)
 {
     ctlab_plan* S = new ctlab_plan();
     S->in = P->in;
@@ -104,7 +77,9 @@ ctlab_plan* subset_plan(const ctlab_plan* P, const std::vector<int>& views)
         S->in.zpos[k] = P->in.zpos[views[k]];
     }
     S->sat_bits = P->sat_bits;
-    return finish_plan(S, P->device);
+    S->tile_nu = P->tile_nu;
+    S->tile_nm = P->tile_nm;
+    return finish_plan(S, P->device, share_ws);
 }
 
 double read_d(const double* d, hipStream_t s, int idx = 0)
@@ -166,21 +141,22 @@ ctlab_recon* ctlab_recon_create(const ctlab_plan* plan, int osNum)
     R->nvox = (long)R->XN * R->YN * R->ZN;
     const long cell = (long)plan->in.DNU * plan->in.DNV;
     R->nprj = (long)plan->in.PN * cell;
+    std::vector<int> all(plan->in.PN);
+    for (int p = 0; p < plan->in.PN; ++p) all[p] = p;
+    R->full = subset_plan(plan, all, nullptr);
+    if (!R->full) { destroy(R); return nullptr; }
+    // the subset plans run one at a time: they borrow the full plan's workspace
     long off = 0;
     for (int k = 0; k < osNum; ++k) {
         std::vector<int> views;
         for (int p = k; p < plan->in.PN; p += osNum) views.push_back(p);
-        ctlab_plan* S = subset_plan(plan, views);
+        ctlab_plan* S = subset_plan(plan, views, R->full);
         if (!S) { destroy(R); return nullptr; }
         R->sub.push_back(S);
         R->sub_off.push_back(off);
         off += (long)views.size() * cell;
         R->perm.insert(R->perm.end(), views.begin(), views.end());
     }
-    std::vector<int> all(plan->in.PN);
-    for (int p = 0; p < plan->in.PN; ++p) all[p] = p;
-    R->full = subset_plan(plan, all);
-    if (!R->full) { destroy(R); return nullptr; }
     const size_t fb = sizeof(float);
     if (chk(hipMalloc(&R->d_b, R->nprj * fb), "hipMalloc") || chk(hipMalloc(&R->d_bperm, R->nprj * fb), "hipMalloc") ||
         chk(hipMalloc(&R->d_rowsum, R->nprj * fb), "hipMalloc") ||
@@ -455,6 +431,12 @@ int ctlab_fista_momentum(float* d_cur, const float* d_las, float t1, float t2, f
 {
     clear_error();
     return chk(rk_fista(d_cur, d_las, t1, t2, minV, maxV, n, (hipStream_t)stream), "fista") ? ctlab_last_error() : 0;
+}
+
+int ctlab_axpby(float a, const float* d_x, float b, float* d_y, long long n, void* stream)
+{
+    clear_error();
+    return chk(rk_axpby(a, d_x, b, d_y, n, (hipStream_t)stream), "axpby") ? ctlab_last_error() : 0;
 }
 
 int ctlab_soft_threshold(float* d_x, float g, long long n, void* stream)

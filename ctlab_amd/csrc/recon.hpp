@@ -1,0 +1,36 @@
+// Internal reconstruction object shared by the library's translation units.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <vector>
+
+#include "plan.hpp"
+
+struct ctlab_recon {
+    int device = 0;
+    int osNum = 1;
+    long nvox = 0, nprj = 0;
+    int XN = 0, YN = 0, ZN = 0;
+    std::vector<ctlab_plan*> sub;       // subset plans (interleaved views)
+    std::vector<long> sub_off;          // element offset of each subset in the permuted projections
+    std::vector<int> perm;              // permuted position -> original view
+    ctlab_plan* full = nullptr;         // all views, original order (FISTA / CG)
+    float* d_b = nullptr;               // measured projections, original order
+    float* d_bperm = nullptr;           // measured projections, subset order
+    float* d_rowsum = nullptr;          // per subset A_s * 1 (subset order)
+    float* d_colsum = nullptr;          // per subset A_s^T * 1 (osNum volumes)
+    float* d_prj = nullptr;             // projection work buffer (nprj)
+    float* d_vol = nullptr;             // volume work buffers
+    float* d_vol2 = nullptr;
+    float* d_vol3 = nullptr;
+    float* d_vol4 = nullptr;
+    float* d_tv = nullptr;              // 7 * nvox for the FGP prox
+    double* d_part = nullptr;           // reduction partials
+    double* d_red = nullptr;            // reduction results (4 doubles)
+    bool have_data = false;
+    hipGraphExec_t sart_graph = nullptr;
+    hipStream_t graph_stream = nullptr;
+    float graph_lambda = 0.0f;
+    int graph_nonneg = -1;
+    float* graph_x = nullptr;           // volume the captured graph updates in place
+};

@@ -114,14 +114,15 @@ class Plan:
 
     def autotune(self, vol, prj):
         """Times the tile variants on (vol, prj) -- both overwritten -- and keeps
-        the fastest; returns {'proj': {8: ms, 16: ms, 32: ms}, 'back': {...}}."""
+        the fastest; returns {'proj': {8: ms, 12: ms, 16: ms}, 'back': {...}}."""
         ms = np.zeros(6, dtype=np.float32)
         self._check_tensor(vol, self.g.nvox(), "volume")
         self._check_tensor(prj, self.g.nprj(), "projection")
         rc = _lib.lib().ctlab_plan_autotune(self.h, C.c_void_p(vol.data_ptr()), C.c_void_p(prj.data_ptr()),
                                             ms.ctypes.data_as(C.POINTER(C.c_float)))
         _lib.check(rc)
-        return {"proj": dict(zip((8, 16, 32), ms[:3].tolist())), "back": dict(zip((8, 16, 32), ms[3:].tolist()))}
+        return {"proj": dict(zip((8, 12, 16), ms[:3].tolist())),
+                "back": dict(zip((8, 12, 16), ms[3:].tolist()))}
 
     def views(self):
         a, b = C.c_int(), C.c_int()

@@ -77,6 +77,58 @@ class Recon:
             pass
 
 
+class Batch:
+    """Many OS-SART reconstructions advanced together (ctlab_batch_*): every
+    phase of a subset step is one persistent launch over all problems and one
+    iteration is a HIP graph.  Results equal the per-problem Recon.os_sart."""
+
+    def __init__(self, recons):
+        self.recons = list(recons)
+        arr = (C.c_void_p * len(self.recons))(*[r.h.value for r in self.recons])
+        h = _lib.lib().ctlab_batch_create(arr, len(self.recons))
+        if not h:
+            _lib.check()
+            raise _lib.CtlabError(-1, "batch creation failed")
+        self.h = C.c_void_p(h)
+
+    def _vols(self, xs):
+        if len(xs) != len(self.recons):
+            raise ValueError(f"need {len(self.recons)} volumes, got {len(xs)}")
+        return (C.c_void_p * len(xs))(*[x.data_ptr() for x in xs])
+
+    def set_tiles(self, nu, nm):
+        _lib.check(_lib.lib().ctlab_batch_set_tiles(self.h, nu, nm))
+
+    def tiles(self):
+        a, b = C.c_int(), C.c_int()
+        _lib.check(_lib.lib().ctlab_batch_get_tiles(self.h, C.byref(a), C.byref(b)))
+        return a.value, b.value
+
+    def os_sart(self, xs, iters, lam=1.0, nonneg=False, use_graph=True, stream=None):
+        _lib.check(_lib.lib().ctlab_batch_os_sart(self.h, self._vols(xs), iters, lam, int(nonneg),
+                                                  int(use_graph), _stream_handle(stream)))
+        return xs
+
+    def autotune(self, xs, lam=1.0):
+        """Times one iteration per tile size on xs (updated); keeps the fastest.
+        Returns {'proj': {8: ms, 12: ms, 16: ms}, 'back': {...}}."""
+        ms = np.zeros(6, dtype=np.float32)
+        _lib.check(_lib.lib().ctlab_batch_autotune(self.h, self._vols(xs), lam,
+                                                   ms.ctypes.data_as(C.POINTER(C.c_float))))
+        return {"proj": dict(zip((8, 12, 16), ms[:3].tolist())), "back": dict(zip((8, 12, 16), ms[3:].tolist()))}
+
+    def close(self):
+        if getattr(self, "h", None):
+            _lib.lib().ctlab_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 # ---- elementwise kernels (reference functors) ------------------------------
 def sart_prj_weight(prj, real, rowsum, stream=None):
     _lib.check(_lib.lib().ctlab_sart_prj_weight(_p(prj), _p(real), _p(rowsum), prj.numel(), _stream_handle(stream)))
@@ -100,6 +152,12 @@ def fista_momentum(cur, las, t1, t2, minV=-2000.0, maxV=6000.0, stream=None):
     return cur
 
 
+def axpby(a, x, b, y, stream=None):
+    """y = a*x + b*y"""
+    _lib.check(_lib.lib().ctlab_axpby(a, _p(x), b, _p(y), y.numel(), _stream_handle(stream)))
+    return y
+
+
 def soft_threshold(x, g, stream=None):
     _lib.check(_lib.lib().ctlab_soft_threshold(_p(x), g, x.numel(), _stream_handle(stream)))
     return x
@@ -110,6 +168,13 @@ def cg_beta(gk, gk1, d, cases, stream=None):
     _lib.check(_lib.lib().ctlab_cg_beta(_p(gk), _p(gk1), _p(d), cases, gk.numel(),
                                         out.ctypes.data_as(C.POINTER(C.c_double)), _stream_handle(stream)))
     return out[0] / out[1]
+
+
+def tv_prox_into(b, x, lam, iters, shape, work, stream=None):
+    """x = TV prox of b with a caller-owned work buffer of 7*numel floats."""
+    YN, XN, ZN = shape
+    _lib.check(_lib.lib().ctlab_tv_prox(_p(b), _p(x), lam, iters, XN, YN, ZN, _p(work), _stream_handle(stream)))
+    return x
 
 
 def tv_prox(b, lam, iters, shape, stream=None):

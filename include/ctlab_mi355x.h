@@ -85,12 +85,12 @@ int ctlab_plan_set_sat_bits(ctlab_plan* plan, int bits);
 int ctlab_proj(ctlab_plan* plan, const float* d_vol, float* d_prj, void* stream);
 int ctlab_back(ctlab_plan* plan, const float* d_prj, float* d_vol, void* stream);
 /* tile shapes: detector columns per projector wavefront (nu) and voxel rows
- * per backprojector wavefront (nm), each 8, 16 or 32 (default 16 / 16) */
+ * per backprojector wavefront (nm), each 8, 12 or 16 (default 16 / 16) */
 int ctlab_plan_set_tiles(ctlab_plan* plan, int nu, int nm);
 int ctlab_plan_get_tiles(const ctlab_plan* plan, int* nu, int* nm);
 /* autotuner: times every tile variant of both operators on these device
  * buffers (contents overwritten) and keeps the fastest; ms_out (6 floats or
- * NULL) receives proj 8/16/32 then back 8/16/32 milliseconds */
+ * NULL) receives proj 8/12/16 then back 8/12/16 milliseconds */
 int ctlab_plan_autotune(ctlab_plan* plan, float* d_vol, float* d_prj, float* ms_out);
 /* geometry queries */
 int ctlab_plan_views(const ctlab_plan* plan, int* n_yslab, int* n_xslab);
@@ -126,6 +126,30 @@ double ctlab_recon_gradient(ctlab_recon* recon, const float* d_y, float* d_g, vo
 /* largest eigenvalue of A^T A (power iteration) */
 double ctlab_recon_lipschitz(ctlab_recon* recon, int iters, void* stream);
 
+/* ---- batched OS-SART (this library's own; the reference runs one problem per
+ * call, src/DD3_GPU_recon.h:60-65) -----------------------------------------
+ * n reconstructions (each with data set, one device, one SAT precision; any
+ * volume, view and detector sizes and subset counts) advance together: every
+ * phase of a subset step is one persistent launch over all problems, and one
+ * iteration is captured in a HIP graph.  Results equal n separate
+ * ctlab_os_sart calls bit for bit.  The batch borrows the reconstructions'
+ * buffers: do not run them on their own while the batch is in use, and
+ * destroy the batch first. */
+typedef struct ctlab_batch ctlab_batch;
+ctlab_batch* ctlab_batch_create(ctlab_recon* const* recons, int n);
+void ctlab_batch_destroy(ctlab_batch* batch);
+int ctlab_batch_size(const ctlab_batch* batch);
+/* tiles of the batched projector (nu) and backprojector (nm): 8, 12 or 16 */
+int ctlab_batch_set_tiles(ctlab_batch* batch, int nu, int nm);
+int ctlab_batch_get_tiles(const ctlab_batch* batch, int* nu, int* nm);
+/* iters OS-SART iterations on d_x[0..n-1] (device volumes, updated in place) */
+int ctlab_batch_os_sart(ctlab_batch* batch, float* const* d_x, int iters, float lambda, int nonneg,
+                        int use_graph, void* stream);
+/* runs one iteration per tile size on d_x (updated), keeps the fastest
+ * projector and backprojector tiles; ms_out (6 floats or NULL): projection
+ * phases for nu = 8/12/16, then backprojection phases for nm = 8/12/16 */
+int ctlab_batch_autotune(ctlab_batch* batch, float* const* d_x, float lambda, float* ms_out);
+
 /* elementwise / reduction kernels on device buffers (stream-ordered):
  * SART_Weighting::Proj / Back (src/SARTWeighting.h:97-136), ossart_update_functor
  * (src/utilities.hpp:1202-1223), FISTA_functor (src/FastIterativeShrinkageAlgorithm.h:
@@ -138,6 +162,8 @@ int ctlab_ossart_update(float* d_img, const float* d_cor, const float* d_weg, co
 int ctlab_fista_momentum(float* d_cur, const float* d_las, float t1, float t2, float minV, float maxV,
                          long long n, void* stream);
 int ctlab_soft_threshold(float* d_x, float g, long long n, void* stream);
+/* d_y = a*d_x + b*d_y (the CG_update1/2/3 form, src/utilities.hpp:1121-1156) */
+int ctlab_axpby(float a, const float* d_x, float b, float* d_y, long long n, void* stream);
 int ctlab_cg_beta(const float* d_gk, const float* d_gk1, const float* d_d, int cases, long long n,
                   double* h_numden, void* stream);
 /* isotropic TV prox (FGP, iters steps) on a (Z,X,Y) volume; d_work: 7*XN*YN*ZN floats */

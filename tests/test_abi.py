@@ -89,3 +89,20 @@ def test_error_strings():
     assert L.ctlab_error_string(0) == b"success"
     assert L.ctlab_error_string(2) == b"unsupported projection mode"
     assert L.ctlab_error_string(99) == b"unknown error"
+
+
+def test_plan_recon_batch_argument_checks():
+    """Entry points beyond the reference ABI reject bad arguments before any device call."""
+    L = _lib.lib()
+    assert L.ctlab_plan_set_tiles(None, 16, 16) == 1
+    assert not L.ctlab_recon_create(None, 1)
+    assert _lib.last_error()[0] == 1
+    assert not L.ctlab_batch_create(None, 0)
+    assert _lib.last_error()[0] == 1
+    arr = (C.c_void_p * 2)(None, None)
+    assert not L.ctlab_batch_create(arr, 2)
+    assert _lib.last_error()[0] == 1 and "reconstruction 0" in _lib.last_error()[1]
+    assert L.ctlab_batch_set_tiles(None, 24, 8) == 1
+    assert L.ctlab_batch_os_sart(None, None, 1, 1.0, 0, 1, None) == 1
+    assert L.ctlab_batch_autotune(None, None, 1.0, None) == 1
+    assert L.ctlab_batch_size(None) == 0

@@ -68,7 +68,7 @@ def test_back_parity_cone(cuda, arc):
     assert e32 <= TOL32
 
 
-@pytest.mark.parametrize("nu,nm", [(8, 32), (32, 8), (16, 16)])
+@pytest.mark.parametrize("nu,nm", [(8, 16), (16, 8), (12, 12), (16, 16)])
 def test_tile_variants(cuda, nu, nm):
     import torch
     g = _cone(XN=40, YN=44, ZN=12, PN=24, DNU=75, DNV=12, ang0=0.3)
@@ -84,8 +84,8 @@ def test_tile_variants(cuda, nu, nm):
     assert rel_l2(p.cpu().numpy(), O.proj(g, vol)) <= TOL64
     assert rel_l2(b.cpu().numpy(), O.back(g, prj)) <= TOL64
     ms = pl.autotune(torch.from_numpy(vol).cuda(), torch.empty(g.prj_shape, device="cuda"))
-    assert set(ms["proj"]) == {8, 16, 32} and min(ms["back"].values()) > 0
-    assert pl.tiles()[0] in (8, 16, 32)
+    assert set(ms["proj"]) == {8, 12, 16} and min(ms["back"].values()) > 0
+    assert pl.tiles()[0] in (8, 12, 16)
 
 
 def test_random_volume_and_mask(cuda):
