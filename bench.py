@@ -141,6 +141,107 @@ def learned_bench(args):
     print(json.dumps(res))
 
 
+def cfg4_specs(n=32, seed=4):
+    """n uneven problems: volume 256..512 (step 32) square x 16..64 slices, 360..984
+    views, detector 512..888 x 16..64, I0 log-uniform in [1e4, 1e6]; the first two
+    are the smallest and the largest corner of that range."""
+    rng = np.random.default_rng(seed)
+    specs = []
+    for i in range(n):
+        XN = int(256 + 32 * rng.integers(0, 9))
+        specs.append(dict(XN=XN, YN=XN, ZN=int(16 + 8 * rng.integers(0, 7)), PN=int(rng.integers(360, 985)),
+                          DNU=int(rng.integers(512, 889)), DNV=int(16 + 8 * rng.integers(0, 7)),
+                          I0=float(10 ** rng.uniform(4, 6)), seed=100 + i))
+    specs[0].update(XN=256, YN=256, ZN=16, PN=360, DNU=512, DNV=16, I0=1e4)
+    specs[1].update(XN=512, YN=512, ZN=64, PN=984, DNU=888, DNV=64, I0=1e6)
+    return specs
+
+
+def cfg4_geometry(sp):
+    """cfg2 scanner; slice thickness chosen so the slab fits inside the cone at the isocentre."""
+    from ctlab_amd import geometry as G
+    dz = min(1.0, 0.9 * sp["DNV"] * 1.0963 * 541.0 / 949.0 / sp["ZN"])
+    return G.circular_cone(XN=sp["XN"], YN=sp["YN"], ZN=sp["ZN"], dx=0.5, dz=dz, PN=sp["PN"], DNU=sp["DNU"],
+                           DNV=sp["DNV"], col_size=1.0239, row_size=1.0963, sid=541.0, sdd=949.0)
+
+
+def batch_bench(args):
+    """cfg4: 32 OS-SART problems (10 subsets) in one batch vs one after another."""
+    import torch
+    from ctlab_amd import dd
+    from ctlab_amd import learned as LE
+    from ctlab_amd import phantom as PH
+    from ctlab_amd import recon as RC
+    dev = torch.device("cuda:0")
+    log = lambda m: print(m, file=sys.stderr, flush=True)
+    specs = cfg4_specs(args.problems)
+    probs = []
+    t0 = time.perf_counter()
+    for sp in specs:
+        g = cfg4_geometry(sp)
+        plan = dd.Plan(g)
+        ph = LE.voxelize_torch(PH.random_ellipsoids(sp["seed"], 20, g.XN, g.YN, g.ZN, g.dx, g.dz), g.vol_shape,
+                               g.dx, g.dz, dev)
+        b = LE.noisy_projection(plan, ph, sp["I0"], sp["seed"])
+        rc = RC.Recon(plan, args.subsets)
+        rc.set_data(b)
+        del b
+        probs.append((g, rc, ph))
+    torch.cuda.synchronize()
+    log(f"{len(probs)} problems set up in {time.perf_counter() - t0:.1f} s, "
+        f"{torch.cuda.memory_allocated() / 2**30:.1f} GiB torch + library buffers")
+    work = sum(g.nvox() * g.PN for g, _, _ in probs)      # voxel*views per iteration (proj + back)
+    bt = RC.Batch([rc for _, rc, _ in probs])
+    tune = None
+    if not args.no_tune:
+        scratch = [torch.zeros(g.vol_shape, device=dev) for g, _, _ in probs]
+        tune = bt.autotune(scratch)
+        del scratch
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(4)]
+    # batched: warmup iterations, then the timed ones (graph replays)
+    xb = [torch.zeros(g.vol_shape, device=dev) for g, _, _ in probs]
+    bt.os_sart(xb, args.warmup)
+    torch.cuda.synchronize()
+    ev[0].record()
+    bt.os_sart(xb, args.steps)
+    ev[1].record()
+    torch.cuda.synchronize()
+    t_batch = ev[0].elapsed_time(ev[1]) / args.steps
+    log(f"batched: {t_batch:.1f} ms/iteration")
+    # the same iterations one problem after another (per-problem graphs, same tiles)
+    for _, rc, _ in probs:
+        rc.set_tiles(*bt.tiles())
+    xs = [torch.zeros(g.vol_shape, device=dev) for g, _, _ in probs]
+    for (g, rc, _), x in zip(probs, xs):
+        rc.os_sart(x, args.warmup)
+    torch.cuda.synchronize()
+    ev[2].record()
+    for (g, rc, _), x in zip(probs, xs):
+        rc.os_sart(x, args.steps)
+    ev[3].record()
+    torch.cuda.synchronize()
+    t_seq = ev[2].elapsed_time(ev[3]) / args.steps
+    log(f"sequential: {t_seq:.1f} ms/iteration")
+    same = all(torch.equal(a, b) for a, b in zip(xb, xs))
+    psnr = [LE.psnr(x, ph) for x, (_, _, ph) in zip(xb, probs)]
+    res = {
+        "metric": METRIC, "mode": "cfg4 batched OS-SART", "value": work / (t_batch * 1e-3),
+        "unit": "voxel·views/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": t_batch, "higher_is_better": True, "dtype": "f32 data, f64 integral images",
+        "data": "synthetic: 20 seeded random ellipsoids per problem, Poisson noise at I0, log transform",
+        "config": {"workload": f"cfg4: {len(probs)} OS-SART problems ({args.subsets} subsets) of uneven size, "
+                               "one step = one iteration of every problem",
+                   "problems": [{k: sp[k] for k in ("XN", "YN", "ZN", "PN", "DNU", "DNV", "I0")} for sp in specs],
+                   "voxel_views_per_iteration": work, "tiles": dict(zip(("nu", "nm"), bt.tiles()))},
+        "autotune_ms": tune,
+        "sequential_ms_per_iteration": t_seq, "batched_speedup": t_seq / t_batch,
+        "batched_equals_sequential_bitwise": bool(same),
+        "psnr_after_iterations": {"iterations": args.warmup + args.steps, "mean": float(np.mean(psnr)),
+                                  "min": float(np.min(psnr)), "max": float(np.max(psnr))},
+    }
+    print(json.dumps(res))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -151,6 +252,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-tune", action="store_true")
     ap.add_argument("--learned", action="store_true", help="cfg5: learned FISTA vs plain FISTA-TV")
+    ap.add_argument("--batch", action="store_true", help="cfg4: batched OS-SART over uneven problems")
+    ap.add_argument("--problems", type=int, default=32)
+    ap.add_argument("--subsets", type=int, default=10)
     ap.add_argument("--samples", type=int, default=2000)
     ap.add_argument("--epochs", type=int, default=3)
     ap.add_argument("--iters", type=int, default=60)
@@ -164,6 +268,8 @@ def main():
 
     if args.learned:
         return learned_bench(args)
+    if args.batch:
+        return batch_bench(args)
 
     import torch
     from ctlab_amd import dd

@@ -12,6 +12,7 @@ def declare(L):
         "ctlab_recon_create": ([_vp, i], _vp),
         "ctlab_recon_destroy": ([_vp], None),
         "ctlab_recon_set_data": ([_vp, _fp, _vp], i),
+        "ctlab_recon_set_tiles": ([_vp, i, i], i),
         "ctlab_os_sart": ([_vp, _fp, i, f, i, i, _vp], i),
         "ctlab_fista_tv": ([_vp, _fp, i, f, i, f, i, _dp, _vp], i),
         "ctlab_cg": ([_vp, _fp, i, i, _dp, _vp], i),

@@ -51,6 +51,9 @@ struct BatchProb {
 
 enum Phase { PH_ZPREFIX = 0, PH_PROJ, PH_PREFIX, PH_BACK_Y, PH_BACK_X, PH_COUNT };
 
+// columns per wavefront and group in the streaming phases (z-prefix, v-prefix)
+constexpr int kChunkColumns = 16;
+
 // The descriptors are never written while a phase runs: read them through the
 // constant address space so that they land in scalar registers.
 template <typename T>
@@ -73,25 +76,33 @@ struct Queue {
     const int* off;
     int* counter;
     int nprob;
-    int chunk;      // items taken per atomic
 };
 
-// Pulls chunks of items until the queue is empty and calls f(k, local item).
-// A wavefront's chunks arrive in increasing order, so its problem index only
-// moves forward.  Every wavefront leaves once the counter passes the total.
-template <class F>
-__device__ __forceinline__ void drain(const Queue q, int lane, F&& f)
+// Pulls groups of 4*CHUNK consecutive items per block (one atomic) until the
+// queue is empty; wavefront w takes items base + w*CHUNK ... + CHUNK-1 and
+// calls f(k, local item) for each.  With CHUNK = 1 the four wavefronts of a
+// block work on four neighbouring items at once (neighbouring views or slabs,
+// which read nearly the same data through the CU's L1), as the blocks of the
+// single-problem kernels do.  Groups arrive in increasing order, so each
+// wavefront's problem index only moves forward.  The whole block sees the
+// same group and leaves together once the counter passes the total.
+template <int CHUNK, class F>
+__device__ __forceinline__ void drain(const Queue q, F&& f)
 {
+    __shared__ int s_base;
     const int total = load_const(q.off + q.nprob);
+    const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     int k = 0;
     for (;;) {
-        int base = 0;
-        if (lane == 0) base = atomicAdd(q.counter, q.chunk);
-        base = __shfl(base, 0);
-        base = __builtin_amdgcn_readfirstlane(base);
+        if (threadIdx.x == 0) s_base = atomicAdd(q.counter, 4 * CHUNK);
+        __syncthreads();
+        const int base = __builtin_amdgcn_readfirstlane(s_base);
+        __syncthreads();
         if (base >= total) break;
-        const int end = min(base + q.chunk, total);
-        for (int it = base; it < end; ++it) {
+#pragma unroll
+        for (int c = 0; c < CHUNK; ++c) {
+            const int it = base + wv * CHUNK + c;
+            if (it >= total) break;
             while (it >= load_const(q.off + k + 1)) ++k;
             f(k, it - load_const(q.off + k));
         }
@@ -109,7 +120,7 @@ template <typename T>
 __global__ void __launch_bounds__(256) bk_zprefix(const BatchProb* __restrict__ pb, Queue q)
 {
     const int lane = threadIdx.x & 63;
-    drain(q, lane, [&](int k, int col) {
+    drain<kChunkColumns>(q, [&](int k, int col) {
         const BatchProb b = load_prob(pb + k);
         zprefix_wave(b.x, (CTLAB_GMEM T*)b.cz, b.g.mask, (long)col, b.g.ZN, lane);
     });
@@ -123,7 +134,7 @@ __global__ void __launch_bounds__(256) bk_proj(const BatchProb* __restrict__ pb,
     __shared__ T sG[4][72];
     const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int lane = threadIdx.x & 63;
-    drain(q, lane, [&](int k, int it) {
+    drain<1>(q, [&](int k, int it) {
         const BatchProb b = load_prob(pb + k);
         proj_walk_wave<T, NU>(b.g, (const CTLAB_GMEM T*)b.cz, b.prj, it % b.g.PN, it / b.g.PN, lane, sG[wv]);
     });
@@ -134,7 +145,7 @@ template <typename T>
 __global__ void __launch_bounds__(256) bk_prefix(const BatchProb* __restrict__ pb, Queue q)
 {
     const int lane = threadIdx.x & 63;
-    drain(q, lane, [&](int k, int it) {
+    drain<kChunkColumns>(q, [&](int k, int it) {
         const BatchProb b = load_prob(pb + k);
         prjprefix_wave<T>(b.g, b.prj, (CTLAB_GMEM T*)b.P, (long)it, lane, b.b, b.rowsum);
     });
@@ -151,7 +162,7 @@ __global__ void __launch_bounds__(256) bk_back(const BatchProb* __restrict__ pb,
     __shared__ T sG[4][72];
     const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int lane = threadIdx.x & 63;
-    drain(q, lane, [&](int k, int it) {
+    drain<1>(q, [&](int k, int it) {
         const BatchProb b = load_prob(pb + k);
         const DevGeom& g = b.g;
         const int NS = yslab ? g.YN : g.XN, NR = yslab ? g.XN : g.YN;
@@ -228,8 +239,6 @@ void destroy(ctlab_batch* B)
     if (B->d_cnt) hipFree(B->d_cnt);
     delete B;
 }
-
-int chunk_of(int ph) { return (ph == PH_ZPREFIX || ph == PH_PREFIX) ? 16 : 1; }
 
 // per-problem item counts of every (subset, phase) for the current tiles
 void build_offsets(ctlab_batch* B)
@@ -318,7 +327,6 @@ int enqueue_step(ctlab_batch* B, int k, float lambda, int nonneg, hipStream_t s,
         q.off = B->d_off + ((size_t)k * PH_COUNT + ph) * (n + 1);
         q.counter = B->d_cnt + k * PH_COUNT + ph;
         q.nprob = n;
-        q.chunk = chunk_of(ph);
         return q;
     };
     auto total = [&](int ph) { return B->off[((size_t)k * PH_COUNT + ph) * (n + 1) + n]; };

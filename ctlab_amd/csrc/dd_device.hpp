@@ -104,8 +104,12 @@ __device__ __forceinline__ void proj_walk_wave(const DevGeom& g, const CTLAB_GME
         T c2 = base[min(off + 2 * row_off, last)];
         T S = T(0);
         T Eprev = (T)(ap - mf) * c0;
-        for (int q = 1; q <= nu; ++q) {
-            double aq = vp.rbase + rs * readlane_d(kap_l, vp.asc ? q : nu - q);
+        // constant trip count (boundaries past the tile's last repeat it and
+        // add nothing), so that the loop unrolls and acc[] stays in registers
+#pragma unroll
+        for (int q = 1; q <= NU; ++q) {
+            const int qc = min(q, nu);
+            double aq = vp.rbase + rs * readlane_d(kap_l, vp.asc ? qc : nu - qc);
             aq = fmin(fmax(aq, 0.0), (double)NR);
             while (m < NR - 1 && aq >= mf + 1.0) {
                 S += c0;
@@ -120,7 +124,7 @@ __device__ __forceinline__ void proj_walk_wave(const DevGeom& g, const CTLAB_GME
             const T H = E - Eprev;
             Eprev = E;
             if (aq > ap) {
-                const double t = g.cone ? sj * readlane_d(tau_l, vp.asc ? q - 1 : nu - q) : 1.0;
+                const double t = g.cone ? sj * readlane_d(tau_l, vp.asc ? qc - 1 : nu - qc) : 1.0;
                 G[lane + 1] = H;
                 const T Dhi = lds_interp(G, vp.beta0 + t * zhi, ZN);
                 const T Dlo = lds_interp(G, vp.beta0 + t * zlo, ZN);
@@ -252,9 +256,9 @@ __device__ __forceinline__ void back_walk_accum(const DevGeom& g, const CTLAB_GM
         T Pr1 = Pv[(long)(asc ? c1 : DNU - 1 - c1) * DNV];
         T D = column_delta(g, G, Pr0, r.esdz, inv_sj, vp.beta0, lane);
         T Tsum = T(0), Wprev = T(0);
-#pragma unroll
-        for (int i = 0; i <= NM; ++i) {
-            if (i > nm) break;
+        // not unrolled (each copy would carry its own column walk); the row
+        // index is uniform, so acc[] is indexed through s_set_gpr_idx
+        for (int i = 0; i <= nm; ++i) {
             const double mp = (double)(r0 + i);
             while (ahi <= mp && c < DNU - 1) {
                 Tsum += D;
@@ -269,7 +273,7 @@ __device__ __forceinline__ void back_walk_accum(const DevGeom& g, const CTLAB_GM
             }
             const double fr = fmin(fmax((mp - alo) * iw, 0.0), 1.0);
             const T W = Tsum + (T)fr * D;
-            if (i > 0) acc[i - 1] += W - Wprev;
+            if (i > 0) acc[__builtin_amdgcn_readfirstlane(i - 1)] += W - Wprev;
             Wprev = W;
         }
     }

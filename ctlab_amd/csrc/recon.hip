@@ -188,6 +188,26 @@ ctlab_recon* ctlab_recon_create(const ctlab_plan* plan, int osNum)
 
 void ctlab_recon_destroy(ctlab_recon* R) { destroy(R); }
 
+int ctlab_recon_set_tiles(ctlab_recon* R, int nu, int nm)
+{
+    clear_error();
+    auto ok = [](int t) { return t == 8 || t == 12 || t == 16; };
+    if (!R || !ok(nu) || !ok(nm)) {
+        set_error(CTLAB_ERR_ARG, "tile sizes must be 8, 12 or 16");
+        return CTLAB_ERR_ARG;
+    }
+    DeviceGuard dg(R->device);
+    for (ctlab_plan* P : R->sub) { P->tile_nu = nu; P->tile_nm = nm; }
+    R->full->tile_nu = nu;
+    R->full->tile_nm = nm;
+    if (R->sart_graph) {   // captured with the old kernels
+        if (chk(hipStreamSynchronize(R->graph_stream), "sync")) return ctlab_last_error();
+        hipGraphExecDestroy(R->sart_graph);
+        R->sart_graph = nullptr;
+    }
+    return CTLAB_OK;
+}
+
 int ctlab_recon_set_data(ctlab_recon* R, const float* d_b, void* stream)
 {
     clear_error();

@@ -29,6 +29,9 @@ class Recon:
         self.h = C.c_void_p(h)
         self.osNum = osNum
 
+    def set_tiles(self, nu, nm):
+        _lib.check(_lib.lib().ctlab_recon_set_tiles(self.h, nu, nm))
+
     def set_data(self, b, stream=None):
         _lib.check(_lib.lib().ctlab_recon_set_data(self.h, _p(b), _stream_handle(stream)))
 

@@ -104,6 +104,8 @@ typedef struct ctlab_recon ctlab_recon;
  * sums A_s^T*1 (src/DD3_GPU_recon.h:21-28), and work buffers, all in HBM. */
 ctlab_recon* ctlab_recon_create(const ctlab_plan* plan, int osNum);
 void ctlab_recon_destroy(ctlab_recon* recon);
+/* tiles of every operator the reconstruction runs (see ctlab_plan_set_tiles) */
+int ctlab_recon_set_tiles(ctlab_recon* recon, int nu, int nm);
 /* measured projections (device pointer, plan layout, original view order) */
 int ctlab_recon_set_data(ctlab_recon* recon, const float* d_b, void* stream);
 /* OS-SART (src/DD3_GPU_recon.h:60-65, weighting src/SARTWeighting.h:22-91):

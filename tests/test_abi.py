@@ -106,3 +106,4 @@ def test_plan_recon_batch_argument_checks():
     assert L.ctlab_batch_os_sart(None, None, 1, 1.0, 0, 1, None) == 1
     assert L.ctlab_batch_autotune(None, None, 1.0, None) == 1
     assert L.ctlab_batch_size(None) == 0
+    assert L.ctlab_recon_set_tiles(None, 8, 8) == 1

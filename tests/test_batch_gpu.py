@@ -117,3 +117,11 @@ def test_batch_tiles_and_autotune(cuda, tiles):
     torch.cuda.synchronize()
     for a, b in zip(ref, xs2):
         assert rel_l2(b.cpu().numpy(), a.cpu().numpy()) < 1e-5
+    # per-problem runs on the tuned tiles (graph path, recaptured) match bit for bit
+    for g, _, rc, _ in probs:
+        rc.set_tiles(nu, nm)
+    for (g, _, rc, _), b in zip(probs, xs2):
+        x = torch.zeros(g.vol_shape, device=cuda)
+        rc.os_sart(x, 1, use_graph=True)
+        torch.cuda.synchronize()
+        assert torch.equal(x, b)
