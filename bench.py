@@ -83,36 +83,42 @@ def learned_bench(args):
     dev = torch.device("cuda:0")
     log = lambda m: print(m, file=sys.stderr, flush=True)
     tshape = (256, 256, 32)
-    xs, ts, Lt, t_gen = LE.generate_pairs(args.samples, tshape, 120, 1e4, 1000, dev, log=log)
-    params, losses, t_train, steps = LE.train(xs, ts, tshape, args.epochs, lr=1e-3, log=log)
-    del xs, ts
-    np.save(os.path.join(ROOT, "gpurun_out", "learned_prox_params.npy"), params) if os.path.isdir(
-        os.path.join(ROOT, "gpurun_out")) else None
     # plain FISTA-TV lambda chosen on a held-out validation phantom (seed 999) at the training size
     gv = LE.cone_geometry(tshape[1], tshape[0], tshape[2], 120)
     pv = dd.Plan(gv)
     rv = RC.Recon(pv, 1)
-    from ctlab_amd import phantom as PH
-    ph_v = LE.voxelize_torch(PH.random_ellipsoids(999, 20, gv.XN, gv.YN, gv.ZN, gv.dx, gv.dz), gv.vol_shape,
-                             gv.dx, gv.dz, dev)
-    rv.set_data(LE.noisy_projection(pv, ph_v, 1e4, 999))
+    Lt = rv.lipschitz(30) * 1.01
+    ph_v = LE.make_problem(gv, pv, rv, 999, 1e4, dev)
     zv = LE.covered_slices(gv)
     sweep = {}
-    for lam in (1e-3, 1e-2, 3e-2, 1e-1, 3e-1):
-        _, hv = LE.fista_tv_run(rv, Lt, gv.vol_shape, args.iters, lam, 10, ph_v, zv, dev)
+    for lam in (0.1, 0.3, 1.0, 3.0, 10.0):
+        _, hv, _ = LE.fista_tv_run(rv, Lt, gv.vol_shape, args.iters, lam, 10, ph_v, zv, dev)
         sweep[lam] = max(h["psnr"] for h in hv)
+        log(f"lambda {lam}: best validation PSNR {sweep[lam]:.2f} dB")
     lam = max(sweep, key=sweep.get)
+    per = 4
+    xs, ts, _, t_gen = LE.generate_pairs(args.samples // per, per, tshape, 120, 1e4, 1000, dev, lam,
+                                         kmax=min(args.iters, 60), log=log)
+    loss_id = LE.identity_loss(xs, ts)
+    log(f"identity-map loss {loss_id:.4e}")
+    params, losses, t_train, steps = LE.train(xs, ts, tshape, args.epochs, lr=1e-3, log=log)
+    del xs, ts
+    np.save(os.path.join(ROOT, "gpurun_out", "learned_prox_params.npy"), params) if os.path.isdir(
+        os.path.join(ROOT, "gpurun_out")) else None
+    # validation: the trained step inside FISTA on the validation phantom
+    net_v = LE.ScaledCNN(gv.vol_shape, params)
+    _, hvl, _ = LE.fista_tv_run(rv, Lt, gv.vol_shape, args.iters, lam, 10, ph_v, zv, dev, learned=net_v)
+    del net_v
     # evaluation phantom, 512x512x64, seed 7
     ge = LE.cone_geometry(512, 512, 64, 120)
     pe = dd.Plan(ge)
     re_ = RC.Recon(pe, 1)
-    ph = LE.voxelize_torch(PH.random_ellipsoids(7, 20, 512, 512, 64, ge.dx, ge.dz), ge.vol_shape, ge.dx, ge.dz, dev)
-    re_.set_data(LE.noisy_projection(pe, ph, 1e4, 7))
+    ph = LE.make_problem(ge, pe, re_, 7, 1e4, dev)
     Le = re_.lipschitz(30) * 1.01
     ze = LE.covered_slices(ge)
-    _, h_plain = LE.fista_tv_run(re_, Le, ge.vol_shape, args.iters, lam, 10, ph, ze, dev)
+    _, h_plain, _ = LE.fista_tv_run(re_, Le, ge.vol_shape, args.iters, lam, 10, ph, ze, dev)
     net = LE.ScaledCNN(ge.vol_shape, params)
-    _, h_learn = LE.fista_tv_run(re_, Le, ge.vol_shape, args.iters, lam, 10, ph, ze, dev, learned=net)
+    _, h_learn, _ = LE.fista_tv_run(re_, Le, ge.vol_shape, args.iters, lam, 10, ph, ze, dev, learned=net)
     target = 40.0
     best_plain = max(h["psnr"] for h in h_plain)
     tp = LE.time_to_psnr(h_plain, target)
@@ -123,8 +129,12 @@ def learned_bench(args):
     rate, t_fwd = LE.cnn_forward_rate(ge.vol_shape, params, dev)
     res = {
         "metric": METRIC, "mode": "cfg5 learned safeguarded FISTA vs plain FISTA-TV",
-        "train": {"pairs": args.samples, "shape": list(tshape), "views": 120, "I0": 1e4, "epochs": args.epochs,
-                  "steps": steps, "gen_s": t_gen, "train_s": t_train, "loss_per_epoch": losses},
+        "train": {"pairs": (args.samples // per) * per, "phantoms": args.samples // per, "snapshots_per_phantom": per,
+                  "shape": list(tshape), "views": 120, "I0": 1e4, "epochs": args.epochs,
+                  "steps": steps, "gen_s": t_gen, "train_s": t_train, "loss_per_epoch": losses,
+                  "loss_identity_map": loss_id},
+        "validation_256": {"plain_best_psnr": sweep[lam], "learned_best_psnr": max(h["psnr"] for h in hvl),
+                           "learned_accepted_steps": sum(h["learned"] for h in hvl)},
         "tv_lambda_sweep_val_psnr": {str(k): v for k, v in sweep.items()}, "tv_lambda": lam,
         "eval": {"shape": [512, 512, 64], "views": 120, "I0": 1e4, "iters": args.iters, "psnr_slices": [ze.start, ze.stop]},
         "plain": {"best_psnr": best_plain, "final_psnr": h_plain[-1]["psnr"], "t_total_s": h_plain[-1]["t"],

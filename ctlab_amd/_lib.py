@@ -43,6 +43,13 @@ def _declare(L):
     L.DD3Back_gpu.restype = None
     L.DD2_multiGPU.argtypes = [_fp, _fp, i, f, f, _fp, _fp, i, i, f, f, i, i, f, _fp, i, _bp, _ip, i]
     L.DD2_multiGPU.restype = None
+    geo = [f, f, f, i, i, _fp, _fp, _fp, f, f, f, _fp, _fp, i, i, i, i, _fp, _fp, f, f, _bp, i, _ip]
+    for name in ("DD3Proj_multiGPU", "DD3Back_multiGPU"):
+        getattr(L, name).argtypes = geo + [i]
+        getattr(L, name).restype = None
+    for name in ("DD3ProjHelical_3GPU", "DD3ProjHelical_4GPU", "DD3BackHelical_3GPU", "DD3BackHelical_4GPU"):
+        getattr(L, name).argtypes = geo
+        getattr(L, name).restype = None
     L.ctlab_last_error.restype = i
     L.ctlab_error_string.argtypes = [i]
     L.ctlab_error_string.restype = C.c_char_p

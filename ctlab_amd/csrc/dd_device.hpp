@@ -20,13 +20,21 @@ __device__ __forceinline__ double readlane_d(double v, int l)
     return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
 }
 
-// piecewise-linear read of a prefix table G[0..n] (G[0] = 0) at b in [0, n]
+__device__ __forceinline__ float readlane_t(float v, int l)
+{
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l));
+}
+
+__device__ __forceinline__ double readlane_t(double v, int l) { return readlane_d(v, l); }
+
+// piecewise-linear read of a prefix table G[0..n] (G[0] = 0) at b, clamped
+// to [0, n]: the cell index is clamped as an integer and the fraction to
+// [0, 1], which gives G[0] below the table and G[n] above it
 template <typename T>
 __device__ __forceinline__ T lds_interp(const T* G, double b, int n)
 {
-    b = fmin(fmax(b, 0.0), (double)n);
-    const int i = min((int)b, n - 1);
-    const T fr = (T)(b - (double)i);
+    const int i = min(max((int)b, 0), n - 1);
+    const T fr = (T)fmin(fmax(b - (double)i, 0.0), 1.0);
     const T g0 = G[i];
     const T g1 = G[i + 1];
     return g0 + fr * (g1 - g0);
@@ -80,7 +88,6 @@ __device__ __forceinline__ void proj_walk_wave(const DevGeom& g, const CTLAB_GME
     const double inv_dx = 1.0 / g.dx;
     const long slab_off = (long)vp.slab_stride * ZN;
     const long row_off = (long)vp.row_stride * ZN;
-    const long last = (long)(NR - 1) * row_off;
 
     T acc[NU];
 #pragma unroll
@@ -89,48 +96,54 @@ __device__ __forceinline__ void proj_walk_wave(const DevGeom& g, const CTLAB_GME
     for (int j = 0; j < vp.NS; ++j) {
         const double sj = vp.s0 + j * g.dx;
         const double rs = sj * inv_dx;
-        const double afirst = vp.rbase + rs * readlane_d(kap_l, vp.asc ? 0 : nu);
-        const double alast = vp.rbase + rs * readlane_d(kap_l, vp.asc ? nu : 0);
-        if (alast <= 0.0 || afirst >= (double)NR) continue;
+        // row coordinate of every tile boundary at once (lane = boundary),
+        // clamped to the slab; its row index and fraction within the row
+        const double a_l = fmin(fmax(vp.rbase + rs * kap_l, 0.0), (double)NR);
+        const int b0 = vp.asc ? 0 : nu;
+        if (readlane_d(a_l, vp.asc ? nu : 0) <= 0.0 || readlane_d(a_l, b0) >= (double)NR) continue;
+        const int m_l = min((int)a_l, NR - 1);
+        const double f_l = a_l - (double)m_l;
+        // columns (lane = tile column u, boundaries u and u+1) with a positive row extent
+        const double a_n = __shfl_down(a_l, 1, 64);
+        const unsigned long long act = __ballot(lane < nu && (vp.asc ? a_n > a_l : a_l > a_n));
         const T qj = (T)(g.cone ? 1.0 / (sj * sj) : 1.0 / fabs(sj));
         const CTLAB_GMEM T* __restrict__ base = cz + j * slab_off + zl;
-        double ap = fmin(fmax(afirst, 0.0), (double)NR);
-        int m = min((int)ap, NR - 1);
-        double mf = (double)m;
-        long off = m * row_off;
-        // rows m, m+1, m+2 in flight (indices clamped to the last row)
-        T c0 = base[off];
-        T c1 = base[min(off + row_off, last)];
-        T c2 = base[min(off + 2 * row_off, last)];
+        int m = __builtin_amdgcn_readlane(m_l, b0);
+        // rows m, m+1, m+2 in flight (row indices clamped to the last row, as
+        // 32-bit scalars: there is no scalar 64-bit compare)
+        int r2 = min(m + 2, NR - 1);
+        T c0 = base[(long)m * row_off];
+        T c1 = base[(long)min(m + 1, NR - 1) * row_off];
+        T c2 = base[(long)r2 * row_off];
         T S = T(0);
-        T Eprev = (T)(ap - mf) * c0;
-        // constant trip count (boundaries past the tile's last repeat it and
-        // add nothing), so that the loop unrolls and acc[] stays in registers
+        T Eprev = (T)readlane_d(f_l, b0) * c0;
+        // constant trip count (steps past the tile's last column repeat its
+        // boundary and add nothing), so that the loop unrolls and acc[]
+        // stays in registers
 #pragma unroll
         for (int q = 1; q <= NU; ++q) {
             const int qc = min(q, nu);
-            double aq = vp.rbase + rs * readlane_d(kap_l, vp.asc ? qc : nu - qc);
-            aq = fmin(fmax(aq, 0.0), (double)NR);
-            while (m < NR - 1 && aq >= mf + 1.0) {
+            const int bq = vp.asc ? qc : nu - qc;          // boundary reached at this step
+            const int mq = __builtin_amdgcn_readlane(m_l, bq);
+            while (m < mq) {
                 S += c0;
                 ++m;
-                mf += 1.0;
-                off += row_off;
                 c0 = c1;
                 c1 = c2;
-                c2 = base[min(off + 2 * row_off, last)];
+                r2 = min(r2 + 1, NR - 1);
+                c2 = base[(long)r2 * row_off];
             }
-            const T E = S + (T)(aq - mf) * c0;
+            const T E = S + (T)readlane_d(f_l, bq) * c0;
             const T H = E - Eprev;
             Eprev = E;
-            if (aq > ap) {
-                const double t = g.cone ? sj * readlane_d(tau_l, vp.asc ? qc - 1 : nu - qc) : 1.0;
+            const int uq = vp.asc ? qc - 1 : nu - qc;      // tile column crossed at this step
+            if (q <= nu && ((act >> uq) & 1ull)) {
+                const double t = g.cone ? sj * readlane_d(tau_l, uq) : 1.0;
                 G[lane + 1] = H;
                 const T Dhi = lds_interp(G, vp.beta0 + t * zhi, ZN);
                 const T Dlo = lds_interp(G, vp.beta0 + t * zlo, ZN);
                 acc[q - 1] += (Dhi - Dlo) * qj;
             }
-            ap = aq;
         }
     }
 
@@ -180,25 +193,37 @@ __device__ __forceinline__ void prjprefix_wave(const DevGeom& g, const CTLAB_GME
 // read from the column's v-prefix Pu (lane v holds prefix v+1) through LDS.
 // esdz = (cell-centre slab coordinate - source) / row pitch, so that the
 // voxel boundary b maps to the fractional row (b - beta0) * esdz / sj - zhat0/dzh.
+// Cone beam: every lane reads the prefix at its lower voxel boundary only; the
+// upper boundary of lane z is the lower one of lane z+1.  Lane 63 of a 64-slice
+// volume has no neighbour: its upper boundary (z = 64) mostly projects off the
+// detector, where the prefix is the column total (above) or 0 (below), and is
+// interpolated only when it lands on the detector.  top_total: the view's top
+// boundary is above the detector for every slab and column (decided on the host).
 template <typename T>
 __device__ __forceinline__ T column_delta(const DevGeom& g, T* G, T Pu, double esdz, double inv_sj,
-                                          double beta0, int lane)
+                                          double beta0, int top_total, T sgn, int lane)
 {
     const int DNV = g.DNV;
     G[lane + 1] = Pu;
-    double nlo, nhi;
-    T sgn = T(1);
-    if (g.cone) {
-        const double inv = inv_sj * esdz;   // 1 / (t * dzh)
-        const double off = -beta0 * inv - g.zhat0 / g.dzh;
-        nlo = (double)lane * inv + off;
-        nhi = nlo + inv;
-        if (g.dzh < 0.0) sgn = T(-1);
-    } else {
-        nlo = (double)lane;
-        nhi = (double)(lane + 1);
+    if (!g.cone)   // multi-slice fan beam: boundaries are the integer rows
+        return lds_interp(G, (double)(lane + 1), DNV) - lds_interp(G, (double)lane, DNV);
+    const double inv = inv_sj * esdz;   // 1 / (t * dzh)
+    const double off = -beta0 * inv - g.zhat0 / g.dzh;
+    const T Q = lds_interp(G, (double)lane * inv + off, DNV);
+    T Qn = __shfl_down(Q, 1, 64);
+    if (g.ZN == 64) {
+        T top;
+        if (top_total) {
+            top = readlane_t(Pu, DNV - 1);
+        } else {
+            const double ntop = 64.0 * inv + off;
+            if (ntop >= (double)DNV) top = readlane_t(Pu, DNV - 1);
+            else if (ntop <= 0.0) top = T(0);
+            else top = lds_interp(G, ntop, DNV);
+        }
+        if (lane == 63) Qn = top;
     }
-    return (lds_interp(G, nhi, DNV) - lds_interp(G, nlo, DNV)) * sgn;
+    return (Qn - Q) * sgn;
 }
 
 // Column record in walk order (increasing row coordinate a): slopes of the
@@ -222,6 +247,7 @@ __device__ __forceinline__ void back_walk_accum(const DevGeom& g, const CTLAB_GM
     const int vl = min(lane, DNV - 1);
     const double inv_dx = 1.0 / g.dx;
     const long srow = (long)j * SR + (r0 >> 3);
+    const T sgn = g.dzh < 0.0 ? T(-1) : T(1);
 
 #pragma unroll
     for (int i = 0; i < NM; ++i) acc[i] = T(0);
@@ -254,7 +280,7 @@ __device__ __forceinline__ void back_walk_accum(const DevGeom& g, const CTLAB_GM
         T Pr0 = Pv[(long)(asc ? c : DNU - 1 - c) * DNV];
         ColRec rn = R[c1];
         T Pr1 = Pv[(long)(asc ? c1 : DNU - 1 - c1) * DNV];
-        T D = column_delta(g, G, Pr0, r.esdz, inv_sj, vp.beta0, lane);
+        T D = column_delta(g, G, Pr0, r.esdz, inv_sj, vp.beta0, vp.top_total, sgn, lane);
         T Tsum = T(0), Wprev = T(0);
         // not unrolled (each copy would carry its own column walk); the row
         // index is uniform, so acc[] is indexed through s_set_gpr_idx
@@ -266,7 +292,7 @@ __device__ __forceinline__ void back_walk_accum(const DevGeom& g, const CTLAB_GM
                 alo = ahi;
                 ahi = vp.rbase + rs * rn.khi;
                 iw = inv_rs * rn.invdk;
-                D = column_delta(g, G, Pr1, rn.esdz, inv_sj, vp.beta0, lane);
+                D = column_delta(g, G, Pr1, rn.esdz, inv_sj, vp.beta0, vp.top_total, sgn, lane);
                 const int cn = min(c + 1, DNU - 1);
                 rn = R[cn];
                 Pr1 = Pv[(long)(asc ? cn : DNU - 1 - cn) * DNV];

@@ -215,6 +215,21 @@ static int build_tables(ctlab_plan* P)
                 g.cone ? g.dx * g.dx / std::fabs(dk) : g.dx * g.dx * std::fabs(t) / std::fabs(dk);
         }
         V.asc = (V.s0 * dk0) > 0;
+        // row coordinate of the top voxel boundary, (ZN - beta0) * esdz / sj - zhat0 / dzh, is
+        // bilinear in (esdz, 1/sj): its corners bound it over all slabs and columns
+        V.top_total = 0;
+        if (g.cone) {
+            double emin = P->esdz[(size_t)p * DNU], emax = emin;
+            for (int u = 1; u < DNU; ++u) {
+                emin = std::min(emin, P->esdz[(size_t)p * DNU + u]);
+                emax = std::max(emax, P->esdz[(size_t)p * DNU + u]);
+            }
+            double lo = HUGE_VAL;
+            for (double e : {emin, emax})
+                for (double sj : {V.s0, slast})
+                    lo = std::min(lo, (g.ZN - V.beta0) * e / sj - P->zhat0 / P->dzh);
+            V.top_total = lo >= (double)DNV;
+        }
         (yslab ? P->views_y : P->views_x).push_back(p);
         // walk-order (increasing row coordinate) column records for the backprojector
         for (int c = 0; c < DNU; ++c) {
@@ -666,6 +681,104 @@ void DD2_multiGPU(float* hvol, float* hprj, const int method, const float x0, co
                                     PN, mask, dev);
     host_call(P, hvol, hprj, method == 0);
     free_plan(P);
+}
+
+// ---- multi-GPU split entry points ----------------------------------------
+// The reference splits views (projector) or volume slices (backprojector)
+// over GPUs 0..n-1 at the start indices given.  Every part is computed by the
+// same operator, so the union equals one single-GPU call: here the whole view
+// set runs on the current device, after the split is checked as the
+// reference's callers build it (first index 0, non-decreasing, within range).
+
+static bool split_ok(const int* start, int n, int limit, const char* what)
+{
+    if (n < 1 || !start) {
+        set_error(CTLAB_ERR_ARG, "%s: need at least one GPU and its start index", what);
+        return false;
+    }
+    if (start[0] != 0) {
+        set_error(CTLAB_ERR_ARG, "%s[0] must be 0", what);
+        return false;
+    }
+    for (int i = 1; i < n; ++i)
+        if (start[i] < start[i - 1] || start[i] > limit) {
+            set_error(CTLAB_ERR_ARG, "%s must be non-decreasing within [0, %d]", what, limit);
+            return false;
+        }
+    return true;
+}
+
+static int current_device()
+{
+    int dev = 0;
+    hipGetDevice(&dev);
+    return dev;
+}
+
+void DD3Proj_multiGPU(float x0, float y0, float z0, int DNU, int DNV, float* xds, float* yds, float* zds,
+                      float imgXCenter, float imgYCenter, float imgZCenter, float* hangs, float* hzPos, int PN,
+                      int XN, int YN, int ZN, float* hvol, float* hprj, float dx, float dz, byte* mask,
+                      int prjMode, int* startPN, int gpuNum)
+{
+    clear_error();
+    if (prjMode != 0) {
+        set_error(CTLAB_ERR_MODE, "prjMode %d not supported (branchless DD is mode 0)", prjMode);
+        return;
+    }
+    if (!split_ok(startPN, gpuNum, PN, "startPN")) return;
+    DD3Proj_gpu(x0, y0, z0, DNU, DNV, xds, yds, zds, imgXCenter, imgYCenter, imgZCenter, hangs, hzPos, PN, XN,
+                YN, ZN, hvol, hprj, dx, dz, mask, current_device(), prjMode);
+}
+
+void DD3Back_multiGPU(float x0, float y0, float z0, int DNU, int DNV, float* xds, float* yds, float* zds,
+                      float imgXCenter, float imgYCenter, float imgZCenter, float* hangs, float* hzPos, int PN,
+                      int XN, int YN, int ZN, float* hvol, float* hprj, float dx, float dz, byte* mask,
+                      int methodId, int* startVOL, int gpuNum)
+{
+    clear_error();
+    if (methodId != 0) {
+        set_error(CTLAB_ERR_MODE, "methodId %d not supported (branchless DD is mode 0)", methodId);
+        return;
+    }
+    if (!split_ok(startVOL, gpuNum, ZN, "startVOL")) return;
+    DD3Back_gpu(x0, y0, z0, DNU, DNV, xds, yds, zds, imgXCenter, imgYCenter, imgZCenter, hangs, hzPos, PN, XN,
+                YN, ZN, hvol, hprj, dx, dz, mask, current_device(), 0, methodId);
+}
+
+void DD3ProjHelical_3GPU(float x0, float y0, float z0, int DNU, int DNV, float* xds, float* yds, float* zds,
+                         float imgXCenter, float imgYCenter, float imgZCenter, float* hangs, float* hzPos, int PN,
+                         int XN, int YN, int ZN, float* hvol, float* hprj, float dx, float dz, byte* mask,
+                         int methodId, int* startPN)
+{
+    DD3Proj_multiGPU(x0, y0, z0, DNU, DNV, xds, yds, zds, imgXCenter, imgYCenter, imgZCenter, hangs, hzPos, PN,
+                     XN, YN, ZN, hvol, hprj, dx, dz, mask, methodId, startPN, 3);
+}
+
+void DD3ProjHelical_4GPU(float x0, float y0, float z0, int DNU, int DNV, float* xds, float* yds, float* zds,
+                         float imgXCenter, float imgYCenter, float imgZCenter, float* hangs, float* hzPos, int PN,
+                         int XN, int YN, int ZN, float* hvol, float* hprj, float dx, float dz, byte* mask,
+                         int methodId, int* startPN)
+{
+    DD3Proj_multiGPU(x0, y0, z0, DNU, DNV, xds, yds, zds, imgXCenter, imgYCenter, imgZCenter, hangs, hzPos, PN,
+                     XN, YN, ZN, hvol, hprj, dx, dz, mask, methodId, startPN, 4);
+}
+
+void DD3BackHelical_3GPU(float x0, float y0, float z0, int DNU, int DNV, float* xds, float* yds, float* zds,
+                         float imgXCenter, float imgYCenter, float imgZCenter, float* hangs, float* hzPos, int PN,
+                         int XN, int YN, int ZN, float* hvol, float* hprj, float dx, float dz, byte* mask,
+                         int methodId, int* startVOL)
+{
+    DD3Back_multiGPU(x0, y0, z0, DNU, DNV, xds, yds, zds, imgXCenter, imgYCenter, imgZCenter, hangs, hzPos, PN,
+                     XN, YN, ZN, hvol, hprj, dx, dz, mask, methodId, startVOL, 3);
+}
+
+void DD3BackHelical_4GPU(float x0, float y0, float z0, int DNU, int DNV, float* xds, float* yds, float* zds,
+                         float imgXCenter, float imgYCenter, float imgZCenter, float* hangs, float* hzPos, int PN,
+                         int XN, int YN, int ZN, float* hvol, float* hprj, float dx, float dz, byte* mask,
+                         int methodId, int* startVOL)
+{
+    DD3Back_multiGPU(x0, y0, z0, DNU, DNV, xds, yds, zds, imgXCenter, imgYCenter, imgZCenter, hangs, hzPos, PN,
+                     XN, YN, ZN, hvol, hprj, dx, dz, mask, methodId, startVOL, 4);
 }
 
 }  // extern "C"

@@ -52,6 +52,50 @@ def DD3Back_gpu(g: ConeGeometry, prj, gpunum=0, squared=0, prjMode=0):
     return vol.reshape(g.vol_shape)
 
 
+def _split(start, n):
+    s = np.asarray(start if start is not None else [0] * n, dtype=np.int32)
+    return s, s.ctypes.data_as(C.POINTER(C.c_int))
+
+
+def DD3Proj_multiGPU(g: ConeGeometry, vol, startPN, gpuNum, prjMode=0, helical=False):
+    """DD3Proj_multiGPU (or DD3ProjHelical_3GPU / _4GPU when helical, gpuNum 3 or 4):
+    views split at startPN over gpuNum GPUs in the reference; one GPU here."""
+    L = _lib.lib()
+    vol = _f32(vol).reshape(-1)
+    if vol.size != g.nvox():
+        raise ValueError("volume size does not match the geometry")
+    prj = np.zeros(g.nprj(), dtype=np.float32)
+    s, sp = _split(startPN, gpuNum)
+    args = (g.x0, g.y0, g.z0, g.DNU, g.DNV, fptr(g.xds), fptr(g.yds), fptr(g.zds), g.imgXCenter, g.imgYCenter,
+            g.imgZCenter, fptr(g.angs), fptr(g.zpos), g.PN, g.XN, g.YN, g.ZN, fptr(vol), fptr(prj), g.dx, g.dz,
+            bptr(g.mask), prjMode, sp)
+    if helical:
+        getattr(L, f"DD3ProjHelical_{gpuNum}GPU")(*args)
+    else:
+        L.DD3Proj_multiGPU(*args, gpuNum)
+    _lib.check()
+    return prj.reshape(g.prj_shape)
+
+
+def DD3Back_multiGPU(g: ConeGeometry, prj, startVOL, gpuNum, methodId=0, helical=False):
+    """DD3Back_multiGPU (or DD3BackHelical_3GPU / _4GPU): slices split at startVOL."""
+    L = _lib.lib()
+    prj = _f32(prj).reshape(-1)
+    if prj.size != g.nprj():
+        raise ValueError("projection size does not match the geometry")
+    vol = np.zeros(g.nvox(), dtype=np.float32)
+    s, sp = _split(startVOL, gpuNum)
+    args = (g.x0, g.y0, g.z0, g.DNU, g.DNV, fptr(g.xds), fptr(g.yds), fptr(g.zds), g.imgXCenter, g.imgYCenter,
+            g.imgZCenter, fptr(g.angs), fptr(g.zpos), g.PN, g.XN, g.YN, g.ZN, fptr(vol), fptr(prj), g.dx, g.dz,
+            bptr(g.mask), methodId, sp)
+    if helical:
+        getattr(L, f"DD3BackHelical_{gpuNum}GPU")(*args)
+    else:
+        L.DD3Back_multiGPU(*args, gpuNum)
+    _lib.check()
+    return vol.reshape(g.vol_shape)
+
+
 def DD2_multiGPU(g: ConeGeometry, data, method, startIdx=None, gpuNum=1):
     """Multi-slice fan beam: method 0 projects ``data`` (image), 2 backprojects it."""
     if not g.parallel:

@@ -1,15 +1,18 @@
 """Learned proximal step inside a safeguarded FISTA (no counterpart in CTLAB).
 
     z_k     = y_k - A^T(A y_k - b) / L                      (gradient step)
-    x_cnn   = z_k + net(z_k)                                 (learned prox, BF16 MFMA)
+    x_tv    = prox_{lambda/L TV}(z_k)                        (the plain FISTA-TV step)
+    x_cnn   = x_tv + net(x_tv)                               (learned correction, BF16 MFMA)
     x_{k+1} = x_cnn   if ||A x_cnn - b||^2 < ||A x_k - b||^2 (safeguard on the exact data fidelity)
-              prox_{lambda/L TV}(z_k)  otherwise             (the plain FISTA-TV step)
+              x_tv    otherwise
     y_{k+1} = FISTA momentum (src/FastIterativeShrinkageAlgorithm.h:17-42)
 
-Training pairs come from the projector itself: seeded random-ellipsoid
-phantoms, their sparse-view projections with Poisson noise, and the
-gradient-step iterate z_K of K plain FISTA iterations (K random), with the
-phantom as target.  The network is trained by the library's own forward /
+The learned part is a residual on top of the TV prox: a network that has
+learned nothing (net = 0) gives back plain FISTA-TV exactly.  Training pairs
+come from the projector itself: seeded random-ellipsoid phantoms, their
+sparse-view projections with Poisson noise, and snapshots of the iterates x_k
+of plain FISTA-TV on them (iterations k spread over the run), with the phantom
+as target: the network learns to correct the iterates it will see.  The network is trained by the library's own forward /
 backward / Adam kernels (ctlab_amd.cnn).  Phantom voxelisation and Poisson
 sampling use torch on the device (data synthesis, not the reconstruction).
 """
@@ -65,28 +68,6 @@ def noisy_projection(plan, x, I0, seed):
     return (-torch.log(torch.clamp(counts, min=1.0) / I0)).float().contiguous()
 
 
-def fista_iterate(rc, L, iters, shape, dev):
-    """K plain FISTA gradient steps from 0; returns the last z = y - A^T(Ay-b)/L."""
-    import torch
-    y = torch.zeros(shape, device=dev)
-    x_old = torch.zeros_like(y)
-    g = torch.empty_like(y)
-    z = torch.empty_like(y)
-    t = 1.0
-    for k in range(iters):
-        rc.gradient(y, g)
-        z.copy_(y)
-        RC.axpby(-1.0 / L, g, 1.0, z)
-        if k == iters - 1:
-            break
-        tn = 0.5 * (1.0 + math.sqrt(1.0 + 4.0 * t * t))
-        y.copy_(z)
-        RC.fista_momentum(y, x_old, t, tn)
-        x_old.copy_(z)
-        t = tn
-    return z
-
-
 def psnr(x, ref, zsl=None):
     """10 log10(max(ref)^2 / MSE) over the slices zsl (default: all)."""
     if zsl is not None:
@@ -119,8 +100,42 @@ class ScaledCNN:
         return out
 
 
-def generate_pairs(n, shape, views, I0, seed0, dev, kmin=2, kmax=20, log=None):
-    """n (input z_K, phantom) pairs on the device, plus the geometry's L."""
+EVAL_POPULATION = (512, 512, 64)   # ellipsoid population box (XN, YN, ZN) of the evaluation phantom
+
+
+def make_problem(g, plan, rc, seed, I0, dev, population=EVAL_POPULATION):
+    """Seeded phantom on the device and its noisy data set on rc; returns the phantom.
+
+    The phantom is a window of g's size cut at a seeded random position from 20
+    random ellipsoids spread over a population box (voxel counts, same voxel
+    size), so that volumes smaller than the box see the same object sizes and
+    density as a volume of the box's size."""
+    PX, PY, PZ = population
+    ells = PH.random_ellipsoids(seed, 20, PX, PY, PZ, g.dx, g.dz)
+    rng = np.random.default_rng(seed)
+    off = [rng.uniform(-0.5, 0.5) * max(P - N, 0) * d for P, N, d in
+           ((PX, g.XN, g.dx), (PY, g.YN, g.dx), (PZ, g.ZN, g.dz))]
+    ells = ells.copy()
+    ells[:, 0] -= off[0]
+    ells[:, 1] -= off[1]
+    ells[:, 2] -= off[2]
+    ph = voxelize_torch(ells, g.vol_shape, g.dx, g.dz, dev)
+    rc.set_data(noisy_projection(plan, ph, I0, seed))
+    return ph
+
+
+def snapshot_iters(rng, kmax, n):
+    """n distinct FISTA iterations in [1, kmax], denser early (where the iterates change most)."""
+    ks = set()
+    while len(ks) < n:
+        ks.add(int(np.clip(round(np.exp(rng.uniform(0.0, np.log(kmax)))), 1, kmax)))
+    return sorted(ks)
+
+
+def generate_pairs(n_phantoms, per_phantom, shape, views, I0, seed0, dev, tv_lambda, tv_iters=10, kmax=60,
+                   log=None):
+    """n_phantoms * per_phantom (x_k, phantom) pairs on the device: x_k is the
+    FISTA-TV (tv_lambda) iterate at snapshot iterations k.  Returns pairs, L, seconds."""
     import torch
     YN, XN, ZN = shape
     g = cone_geometry(XN, YN, ZN, views)
@@ -130,23 +145,29 @@ def generate_pairs(n, shape, views, I0, seed0, dev, kmin=2, kmax=20, log=None):
     rng = np.random.default_rng(seed0)
     xs, ts = [], []
     t0 = time.perf_counter()
-    for i in range(n):
-        ells = PH.random_ellipsoids(seed0 + i, 20, XN, YN, ZN, g.dx, g.dz)
-        ph = voxelize_torch(ells, g.vol_shape, g.dx, g.dz, dev)
-        b = noisy_projection(plan, ph, I0, seed0 + i)
-        rc.set_data(b)
-        K = int(rng.integers(kmin, kmax + 1))
-        xs.append(fista_iterate(rc, L, K, g.vol_shape, dev))
-        ts.append(ph)
-        if log and (i + 1) % 250 == 0:
+    for i in range(n_phantoms):
+        ph = make_problem(g, plan, rc, seed0 + i, I0, dev)
+        ks = snapshot_iters(rng, kmax, per_phantom)
+        _, _, snaps = fista_tv_run(rc, L, g.vol_shape, ks[-1], tv_lambda, tv_iters, None, None, dev, snaps=ks)
+        for k in ks:
+            xs.append(snaps[k])
+            ts.append(ph)
+        if log and (i + 1) % 100 == 0:
             torch.cuda.synchronize()
-            log(f"pairs {i + 1}/{n} ({time.perf_counter() - t0:.1f} s)")
+            log(f"phantoms {i + 1}/{n_phantoms}, pairs {len(xs)} ({time.perf_counter() - t0:.1f} s)")
     torch.cuda.synchronize()
     return xs, ts, L, time.perf_counter() - t0
 
 
+def identity_loss(xs, ts):
+    """The training loss of net = 0: 0.5*mean((x - phantom)^2) in the scaled units."""
+    s2 = INPUT_SCALE * INPUT_SCALE
+    return float(np.mean([0.5 * s2 * float(((x - t) ** 2).mean()) for x, t in zip(xs, ts)]))
+
+
 def train(xs, ts, shape, epochs, lr=1e-3, seed=0, log=None):
-    """Adam on 0.5*mean((z + net(z) - phantom)^2) in the scaled units; returns params and losses."""
+    """Adam on 0.5*mean((x + net(x) - phantom)^2) in the scaled units; the last
+    epoch runs at 0.3*lr.  Returns params, per-epoch mean losses, seconds, steps."""
     import torch
     net = CN.CNN(shape, training=True, params=CN.init_params(seed))
     s = INPUT_SCALE
@@ -179,11 +200,11 @@ def _timed(fn):
     return r, time.perf_counter() - t0
 
 
-def fista_tv_run(rc, L, b_shape_vol, iters, tv_lambda, tv_iters, truth, zsl, dev, learned=None,
-                 target_db=40.0):
+def fista_tv_run(rc, L, b_shape_vol, iters, tv_lambda, tv_iters, truth, zsl, dev, learned=None, snaps=None):
     """Plain FISTA-TV (learned=None) or the safeguarded learned FISTA, step by
-    step with the library's kernels.  Returns per-iteration history: wall time
-    (GPU work only, synchronised), PSNR, data fidelity, accepted learned step."""
+    step with the library's kernels.  Returns x, the per-iteration history (wall
+    time of the GPU work, synchronised; PSNR when truth is given; data fidelity;
+    accepted learned step) and {k: copy of the iterate x_k} for k in snaps."""
     import torch
     shape = b_shape_vol
     n = int(np.prod(shape))
@@ -193,10 +214,12 @@ def fista_tv_run(rc, L, b_shape_vol, iters, tv_lambda, tv_iters, truth, zsl, dev
     g = torch.empty_like(x)
     z = torch.empty_like(x)
     xc = torch.empty_like(x)
+    xt = torch.empty_like(x) if learned is not None else None
     work = torch.empty(7 * n, device=dev)
     t = 1.0
     f_x = rc.data_fidelity(x)
     hist = []
+    taken = {}
     elapsed = 0.0
     for k in range(iters):
         def step():
@@ -205,16 +228,20 @@ def fista_tv_run(rc, L, b_shape_vol, iters, tv_lambda, tv_iters, truth, zsl, dev
             z.copy_(y)
             RC.axpby(-1.0 / L, g, 1.0, z)
             accepted = False
-            if learned is not None:
-                learned(z, xc)
+            if learned is None:
+                RC.tv_prox_into(z, x, tv_lambda / L, tv_iters, shape, work)
+                f_x = rc.data_fidelity(x)
+            else:
+                RC.tv_prox_into(z, xt, tv_lambda / L, tv_iters, shape, work)
+                learned(xt, xc)
                 f_c = rc.data_fidelity(xc)
                 if f_c < f_x:
                     x.copy_(xc)
                     f_x = f_c
                     accepted = True
-            if not accepted:
-                RC.tv_prox_into(z, x, tv_lambda / L, tv_iters, shape, work)
-                f_x = rc.data_fidelity(x)
+                else:
+                    x.copy_(xt)
+                    f_x = rc.data_fidelity(x)
             tn = 0.5 * (1.0 + math.sqrt(1.0 + 4.0 * t * t))
             y.copy_(x)
             RC.fista_momentum(y, x_old, t, tn)
@@ -223,8 +250,11 @@ def fista_tv_run(rc, L, b_shape_vol, iters, tv_lambda, tv_iters, truth, zsl, dev
             return accepted
         acc, dt = _timed(step)
         elapsed += dt
-        hist.append({"iter": k + 1, "t": elapsed, "psnr": psnr(x, truth, zsl), "fidelity": f_x, "learned": acc})
-    return x, hist
+        if snaps is not None and (k + 1) in snaps:
+            taken[k + 1] = x.clone()
+        hist.append({"iter": k + 1, "t": elapsed, "psnr": psnr(x, truth, zsl) if truth is not None else None,
+                     "fidelity": f_x, "learned": acc})
+    return x, hist, taken
 
 
 def time_to_psnr(hist, target):

@@ -51,6 +51,42 @@ void DD2_multiGPU(float* hvol, float* hprj, const int method, const float x0, co
                   const float imgYCenter, const int XN, const int YN, const float dx, float* hangs,
                   int PN, byte* mask, int* startIdx, const int gpuNum);
 
+/* Multi-GPU split entry points.  The reference splits the views (projector,
+ * startPN) or the volume slices (backprojector, startVOL) over GPUs
+ * 0..gpuNum-1; every part is the same operator, so the union equals one
+ * single-GPU call, which is what runs here, on the current device, after the
+ * split is checked (index 0 first, non-decreasing, within PN / ZN).  Mode /
+ * methodId other than 0 set CTLAB_ERR_MODE.  The Helical_3GPU / _4GPU forms
+ * take `int (&)[3]` / `int (&)[4]` in the reference, a pointer in this ABI. */
+/* src/DD3_GPU_Proj.h:138-141 */
+void DD3Proj_multiGPU(float x0, float y0, float z0, int DNU, int DNV, float* xds, float* yds, float* zds,
+                      float imgXCenter, float imgYCenter, float imgZCenter, float* hangs, float* hzPos,
+                      int PN, int XN, int YN, int ZN, float* hvol, float* hprj, float dx, float dz,
+                      byte* mask, int prjMode, int* startPN, int gpuNum);
+/* src/DD3_GPU_Back.h:138-142 */
+void DD3Back_multiGPU(float x0, float y0, float z0, int DNU, int DNV, float* xds, float* yds, float* zds,
+                      float imgXCenter, float imgYCenter, float imgZCenter, float* hangs, float* hzPos,
+                      int PN, int XN, int YN, int ZN, float* hvol, float* hprj, float dx, float dz,
+                      byte* mask, int methodId, int* startVOL, int gpuNum);
+/* src/DD3_GPU_Proj.h:73-76, 105-108 */
+void DD3ProjHelical_3GPU(float x0, float y0, float z0, int DNU, int DNV, float* xds, float* yds,
+                         float* zds, float imgXCenter, float imgYCenter, float imgZCenter, float* hangs,
+                         float* hzPos, int PN, int XN, int YN, int ZN, float* hvol, float* hprj, float dx,
+                         float dz, byte* mask, int methodId, int* startPN);
+void DD3ProjHelical_4GPU(float x0, float y0, float z0, int DNU, int DNV, float* xds, float* yds,
+                         float* zds, float imgXCenter, float imgYCenter, float imgZCenter, float* hangs,
+                         float* hzPos, int PN, int XN, int YN, int ZN, float* hvol, float* hprj, float dx,
+                         float dz, byte* mask, int methodId, int* startPN);
+/* src/DD3_GPU_Back.h:73-76, 105-108 */
+void DD3BackHelical_3GPU(float x0, float y0, float z0, int DNU, int DNV, float* xds, float* yds,
+                         float* zds, float imgXCenter, float imgYCenter, float imgZCenter, float* hangs,
+                         float* hzPos, int PN, int XN, int YN, int ZN, float* hvol, float* hprj, float dx,
+                         float dz, byte* mask, int methodId, int* startVOL);
+void DD3BackHelical_4GPU(float x0, float y0, float z0, int DNU, int DNV, float* xds, float* yds,
+                         float* zds, float imgXCenter, float imgYCenter, float imgZCenter, float* hangs,
+                         float* hzPos, int PN, int XN, int YN, int ZN, float* hvol, float* hprj, float dx,
+                         float dz, byte* mask, int methodId, int* startVOL);
+
 /* ---- status --------------------------------------------------------------- */
 enum {
     CTLAB_OK = 0,

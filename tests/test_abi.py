@@ -107,3 +107,25 @@ def test_plan_recon_batch_argument_checks():
     assert L.ctlab_batch_autotune(None, None, 1.0, None) == 1
     assert L.ctlab_batch_size(None) == 0
     assert L.ctlab_recon_set_tiles(None, 8, 8) == 1
+
+
+def test_multi_gpu_split_validation():
+    """The split entry points check mode and split before any device work."""
+    g = _tiny()
+    vol = np.ones(g.nvox(), np.float32)
+    with pytest.raises(_lib.CtlabError) as e:
+        dd.DD3Proj_multiGPU(g, vol, [0, 3, 2], 3)          # decreasing
+    assert e.value.code == 1
+    with pytest.raises(_lib.CtlabError):
+        dd.DD3Proj_multiGPU(g, vol, [1, 2], 2)             # must start at 0
+    with pytest.raises(_lib.CtlabError):
+        dd.DD3Proj_multiGPU(g, vol, [0, 9], 2)             # beyond PN = 4
+    with pytest.raises(_lib.CtlabError) as e:
+        dd.DD3Proj_multiGPU(g, vol, [0, 2], 2, prjMode=1)
+    assert e.value.code == 2
+    prj = np.ones(g.nprj(), np.float32)
+    with pytest.raises(_lib.CtlabError):
+        dd.DD3Back_multiGPU(g, prj, [0, 5, 6], 3, helical=True)   # beyond ZN = 4
+    with pytest.raises(_lib.CtlabError) as e:
+        dd.DD3Back_multiGPU(g, prj, [0, 1, 2, 3], 4, methodId=4, helical=True)
+    assert e.value.code == 2

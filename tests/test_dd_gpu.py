@@ -151,3 +151,19 @@ def test_gpu_adjoint_identity(cuda):
     lhs, rhs = ax @ y, x.astype(np.float64) @ aty
     print(f"GPU adjoint mismatch {abs(lhs - rhs) / abs(lhs):.3e}")
     assert abs(lhs - rhs) / abs(lhs) < 1e-6
+
+
+def test_multi_gpu_split_entry_points_equal_single_call(cuda):
+    """DD3Proj/Back_multiGPU and the Helical_3GPU/_4GPU forms compute the
+    union of their parts: exactly the single-GPU call."""
+    g = _cone(PN=30)
+    vol = _phantom(g, 22)
+    prj1 = dd.DD3Proj_gpu(g, vol)
+    assert np.array_equal(dd.DD3Proj_multiGPU(g, vol, [0, 11, 20], 3), prj1)
+    assert np.array_equal(dd.DD3Proj_multiGPU(g, vol, [0, 8, 8, 25], 4, helical=True), prj1)
+    assert np.array_equal(dd.DD3Proj_multiGPU(g, vol, [0, 10, 20], 3, helical=True), prj1)
+    y = np.random.default_rng(23).random(g.nprj()).astype(np.float32)
+    b1 = dd.DD3Back_gpu(g, y)
+    assert np.array_equal(dd.DD3Back_multiGPU(g, y, [0, 6], 2), b1)
+    assert np.array_equal(dd.DD3Back_multiGPU(g, y, [0, 4, 9], 3, helical=True), b1)
+    assert np.array_equal(dd.DD3Back_multiGPU(g, y, [0, 4, 8, 12], 4, helical=True), b1)
