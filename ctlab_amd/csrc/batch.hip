@@ -131,7 +131,7 @@ __global__ void __launch_bounds__(256) bk_zprefix(const BatchProb* __restrict__ 
 template <typename T, int NU>
 __global__ void __launch_bounds__(256) bk_proj(const BatchProb* __restrict__ pb, Queue q)
 {
-    __shared__ T sG[4][72];
+    __shared__ T sG[4][NU * kGStride];
     const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int lane = threadIdx.x & 63;
     drain<1>(q, [&](int k, int it) {

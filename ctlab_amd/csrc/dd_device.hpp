@@ -27,6 +27,20 @@ __device__ __forceinline__ float readlane_t(float v, int l)
 
 __device__ __forceinline__ double readlane_t(double v, int l) { return readlane_d(v, l); }
 
+// lane i receives lane i+1's value (DPP wave_shl:1, a VALU move); lane 63 receives 0
+__device__ __forceinline__ double shift_down1(double v)
+{
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_update_dpp(0, (int)(b & 0xffffffffLL), 0x130, 0xf, 0xf, true);
+    const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), 0x130, 0xf, 0xf, true);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+
+__device__ __forceinline__ float shift_down1(float v)
+{
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x130, 0xf, 0xf, true));
+}
+
 // piecewise-linear read of a prefix table G[0..n] (G[0] = 0) at b, clamped
 // to [0, n]: the cell index is clamped as an integer and the fraction to
 // [0, 1], which gives G[0] below the table and G[n] above it
@@ -66,25 +80,40 @@ __device__ __forceinline__ void zprefix_wave(const CTLAB_GMEM float* __restrict_
     if (lane < ZN) cz[col * ZN + lane] = r;
 }
 
+// LDS per projector wavefront: one z-prefix table (entries 0..64) per column
+// step of the tile, kGStride entries apart
+constexpr int kGStride = 68;
+
 // Projector walk of one (view p, tile of NU detector columns) over all slabs.
-// Lane = z for the SAT rows, lane = v for outputs; G: 65+ doubles of LDS.
+// Lane = z for the SAT rows, lane = detector row boundary for the outputs;
+// G: NU * kGStride entries of LDS.
+//
+// Row v needs the footprint z-prefix at its two boundaries v and v+1: each lane
+// interpolates boundary `lane` and takes the upper one from lane+1.  With 64
+// rows the top boundary (64) has no lane: its table stays in LDS (one region
+// per column step) and after the slab's column loop one pass with lane = step
+// interpolates all of them; row 63 collects that term lane-parallel in
+// acc_top and adds it in the epilogue.
 template <typename T, int NU>
 __device__ __forceinline__ void proj_walk_wave(const DevGeom& g, const CTLAB_GMEM T* __restrict__ cz,
                                                CTLAB_GMEM float* __restrict__ prj,
                                                int p, int tile, int lane, T* G)
 {
-    if (lane == 0) G[0] = T(0);
+    if (lane < NU) G[lane * kGStride] = T(0);
     const ViewParams vp = g.vp[p];
     const int u0 = tile * NU;
     const int nu = min(NU, g.DNU - u0);
-    // this tile's boundary slopes (lanes 0..nu) and magnifications (lanes 0..nu-1)
-    // are the same for every slab: load once, read per column with v_readlane
+    const int ZN = g.ZN, NR = vp.NR, DNV = g.DNV;
+    // this tile's boundary slopes (lanes 0..nu) and magnifications, by column
+    // (lanes 0..nu-1) and by column step (tau_w), are the same for every slab
     const double kap_l = g.kap[(long)p * (g.DNU + 1) + u0 + min(lane, nu)];
     const double tau_l = g.tau[(long)p * g.DNU + u0 + min(lane, nu - 1)];
-    const int ZN = g.ZN, NR = vp.NR, DNV = g.DNV;
+    const int uw = min(lane, nu - 1);
+    const double tau_w = g.tau[(long)p * g.DNU + u0 + (vp.asc ? uw : nu - 1 - uw)];
+    const bool defer_top = DNV == 64;
     const int zl = min(lane, ZN - 1);
-    const int vl = min(lane, DNV - 1);
-    const double zlo = g.zhat[vl], zhi = g.zhat[vl + 1];
+    const double zb = g.zhat[min(lane, DNV)];        // boundary `lane`
+    const double ztop = g.zhat[DNV];
     const double inv_dx = 1.0 / g.dx;
     const long slab_off = (long)vp.slab_stride * ZN;
     const long row_off = (long)vp.row_stride * ZN;
@@ -92,6 +121,7 @@ __device__ __forceinline__ void proj_walk_wave(const DevGeom& g, const CTLAB_GME
     T acc[NU];
 #pragma unroll
     for (int q = 0; q < NU; ++q) acc[q] = T(0);
+    T acc_top = T(0);   // lane s: row 63's top-boundary term of column step s
 
     for (int j = 0; j < vp.NS; ++j) {
         const double sj = vp.s0 + j * g.dx;
@@ -139,11 +169,27 @@ __device__ __forceinline__ void proj_walk_wave(const DevGeom& g, const CTLAB_GME
             const int uq = vp.asc ? qc - 1 : nu - qc;      // tile column crossed at this step
             if (q <= nu && ((act >> uq) & 1ull)) {
                 const double t = g.cone ? sj * readlane_d(tau_l, uq) : 1.0;
-                G[lane + 1] = H;
-                const T Dhi = lds_interp(G, vp.beta0 + t * zhi, ZN);
-                const T Dlo = lds_interp(G, vp.beta0 + t * zlo, ZN);
-                acc[q - 1] += (Dhi - Dlo) * qj;
+                T* Gq = G + (q - 1) * kGStride;
+                Gq[lane + 1] = H;
+                const T Db = lds_interp(Gq, vp.beta0 + t * zb, ZN);
+                const T Dn = shift_down1(Db);   // lane 63: 0, its top boundary is deferred
+                acc[q - 1] += (Dn - Db) * qj;
             }
+        }
+        if (defer_top) {
+            // lane s < nu: top boundary of column step s (its table is region s)
+            const int us = vp.asc ? lane : nu - 1 - lane;
+            if (lane < nu && ((act >> us) & 1ull)) {
+                const double t = g.cone ? sj * tau_w : 1.0;
+                acc_top += lds_interp(G + lane * kGStride, vp.beta0 + t * ztop, ZN) * qj;
+            }
+        }
+    }
+    if (defer_top) {
+#pragma unroll
+        for (int q = 0; q < NU; ++q) {
+            const T top = readlane_t(acc_top, q);
+            if (lane == 63) acc[q] += top;
         }
     }
 
@@ -193,37 +239,27 @@ __device__ __forceinline__ void prjprefix_wave(const DevGeom& g, const CTLAB_GME
 // read from the column's v-prefix Pu (lane v holds prefix v+1) through LDS.
 // esdz = (cell-centre slab coordinate - source) / row pitch, so that the
 // voxel boundary b maps to the fractional row (b - beta0) * esdz / sj - zhat0/dzh.
-// Cone beam: every lane reads the prefix at its lower voxel boundary only; the
-// upper boundary of lane z is the lower one of lane z+1.  Lane 63 of a 64-slice
-// volume has no neighbour: its upper boundary (z = 64) mostly projects off the
-// detector, where the prefix is the column total (above) or 0 (below), and is
-// interpolated only when it lands on the detector.  top_total: the view's top
-// boundary is above the detector for every slab and column (decided on the host).
+// Both boundaries are interpolated by every lane: two independent LDS reads
+// (sharing the upper boundary with the next lane through a lane shift puts the
+// shift in series after the read, which measured slower here).
 template <typename T>
 __device__ __forceinline__ T column_delta(const DevGeom& g, T* G, T Pu, double esdz, double inv_sj,
-                                          double beta0, int top_total, T sgn, int lane)
+                                          double beta0, T sgn, int lane)
 {
     const int DNV = g.DNV;
     G[lane + 1] = Pu;
-    if (!g.cone)   // multi-slice fan beam: boundaries are the integer rows
-        return lds_interp(G, (double)(lane + 1), DNV) - lds_interp(G, (double)lane, DNV);
-    const double inv = inv_sj * esdz;   // 1 / (t * dzh)
-    const double off = -beta0 * inv - g.zhat0 / g.dzh;
-    const T Q = lds_interp(G, (double)lane * inv + off, DNV);
-    T Qn = __shfl_down(Q, 1, 64);
-    if (g.ZN == 64) {
-        T top;
-        if (top_total) {
-            top = readlane_t(Pu, DNV - 1);
-        } else {
-            const double ntop = 64.0 * inv + off;
-            if (ntop >= (double)DNV) top = readlane_t(Pu, DNV - 1);
-            else if (ntop <= 0.0) top = T(0);
-            else top = lds_interp(G, ntop, DNV);
-        }
-        if (lane == 63) Qn = top;
+    double nlo, nhi;
+    if (g.cone) {
+        const double inv = inv_sj * esdz;   // 1 / (t * dzh)
+        const double off = -beta0 * inv - g.zhat0 / g.dzh;
+        nlo = (double)lane * inv + off;
+        nhi = nlo + inv;
+    } else {
+        nlo = (double)lane;
+        nhi = (double)(lane + 1);
+        sgn = T(1);
     }
-    return (Qn - Q) * sgn;
+    return (lds_interp(G, nhi, DNV) - lds_interp(G, nlo, DNV)) * sgn;
 }
 
 // Column record in walk order (increasing row coordinate a): slopes of the
@@ -280,7 +316,7 @@ __device__ __forceinline__ void back_walk_accum(const DevGeom& g, const CTLAB_GM
         T Pr0 = Pv[(long)(asc ? c : DNU - 1 - c) * DNV];
         ColRec rn = R[c1];
         T Pr1 = Pv[(long)(asc ? c1 : DNU - 1 - c1) * DNV];
-        T D = column_delta(g, G, Pr0, r.esdz, inv_sj, vp.beta0, vp.top_total, sgn, lane);
+        T D = column_delta(g, G, Pr0, r.esdz, inv_sj, vp.beta0, sgn, lane);
         T Tsum = T(0), Wprev = T(0);
         // not unrolled (each copy would carry its own column walk); the row
         // index is uniform, so acc[] is indexed through s_set_gpr_idx
@@ -292,7 +328,7 @@ __device__ __forceinline__ void back_walk_accum(const DevGeom& g, const CTLAB_GM
                 alo = ahi;
                 ahi = vp.rbase + rs * rn.khi;
                 iw = inv_rs * rn.invdk;
-                D = column_delta(g, G, Pr1, rn.esdz, inv_sj, vp.beta0, vp.top_total, sgn, lane);
+                D = column_delta(g, G, Pr1, rn.esdz, inv_sj, vp.beta0, sgn, lane);
                 const int cn = min(c + 1, DNU - 1);
                 rn = R[cn];
                 Pr1 = Pv[(long)(asc ? cn : DNU - 1 - cn) * DNV];

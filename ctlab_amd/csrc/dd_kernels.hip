@@ -47,7 +47,7 @@ template <typename T, int NU>
 __global__ void __launch_bounds__(256)
 dd_proj_walk(DevGeom g, const T* __restrict__ cz, float* __restrict__ prj)
 {
-    __shared__ T sG[4][72];
+    __shared__ T sG[4][NU * kGStride];
     const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int p = blockIdx.y * 4 + wv;
     if (p >= g.PN) return;
