@@ -160,6 +160,7 @@ __global__ void __launch_bounds__(256) bk_back(const BatchProb* __restrict__ pb,
 {
     constexpr int yslab = YSLAB;
     __shared__ T sG[4][72];
+    __shared__ T sA[4][NM * 64];
     const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int lane = threadIdx.x & 63;
     drain<1>(q, [&](int k, int it) {
@@ -169,7 +170,7 @@ __global__ void __launch_bounds__(256) bk_back(const BatchProb* __restrict__ pb,
         const int j = it % NS, rt = it / NS;
         T acc[NM];
         back_walk_accum<T, NM>(g, (const CTLAB_GMEM T*)b.P, yslab ? b.views_y : b.views_x, yslab ? b.ny : b.nx, NR, j, rt,
-                               b.rec, b.start, b.SJ, b.SR, lane, sG[wv], acc);
+                               b.rec, b.start, b.SJ, b.SR, lane, sG[wv], sA[wv], acc);
         CTLAB_GMEM float* __restrict__ vol = b.vol;
         if constexpr (YSLAB) {
             back_store_tile<T, NM>(g, acc, 1, NR, j, rt, lane, [&](long idx, float v) { vol[idx] = v; });

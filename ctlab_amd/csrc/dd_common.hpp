@@ -40,6 +40,8 @@ struct ViewParams {
     int slab_stride;  // z-column stride between slabs
     int row_stride;   // z-column stride between rows
     int asc;          // 1 if the row coordinate a grows with detector index u
+    int top_total;    // 1 if, for every slab and column, the top voxel boundary z = ZN maps
+                      // at or above the last detector row boundary (backprojector shortcut)
 };
 
 // Everything a kernel needs, passed by value.

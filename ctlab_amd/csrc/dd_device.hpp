@@ -239,27 +239,37 @@ __device__ __forceinline__ void prjprefix_wave(const DevGeom& g, const CTLAB_GME
 // read from the column's v-prefix Pu (lane v holds prefix v+1) through LDS.
 // esdz = (cell-centre slab coordinate - source) / row pitch, so that the
 // voxel boundary b maps to the fractional row (b - beta0) * esdz / sj - zhat0/dzh.
-// Both boundaries are interpolated by every lane: two independent LDS reads
-// (sharing the upper boundary with the next lane through a lane shift puts the
-// shift in series after the read, which measured slower here).
+// Cone beam: every lane interpolates its lower voxel boundary only and takes
+// the upper one from lane z+1 (DPP shift).  Lane 63 of a 64-slice volume has no
+// neighbour: its upper boundary (z = 64) mostly projects off the detector,
+// where the prefix is the column total (above) or 0 (below), and is
+// interpolated only when it lands on the detector.  top_total: the view's top
+// boundary is above the detector for every slab and column (decided on the host).
 template <typename T>
 __device__ __forceinline__ T column_delta(const DevGeom& g, T* G, T Pu, double esdz, double inv_sj,
-                                          double beta0, T sgn, int lane)
+                                          double beta0, int top_total, T sgn, int lane)
 {
     const int DNV = g.DNV;
     G[lane + 1] = Pu;
-    double nlo, nhi;
-    if (g.cone) {
-        const double inv = inv_sj * esdz;   // 1 / (t * dzh)
-        const double off = -beta0 * inv - g.zhat0 / g.dzh;
-        nlo = (double)lane * inv + off;
-        nhi = nlo + inv;
-    } else {
-        nlo = (double)lane;
-        nhi = (double)(lane + 1);
-        sgn = T(1);
+    if (!g.cone)   // multi-slice fan beam: boundaries are the integer rows
+        return lds_interp(G, (double)(lane + 1), DNV) - lds_interp(G, (double)lane, DNV);
+    const double inv = inv_sj * esdz;   // 1 / (t * dzh)
+    const double off = -beta0 * inv - g.zhat0 / g.dzh;
+    const T Q = lds_interp(G, (double)lane * inv + off, DNV);
+    T Qn = shift_down1(Q);
+    if (g.ZN == 64) {
+        T top;
+        if (top_total) {
+            top = readlane_t(Pu, DNV - 1);
+        } else {
+            const double ntop = 64.0 * inv + off;
+            if (ntop >= (double)DNV) top = readlane_t(Pu, DNV - 1);
+            else if (ntop <= 0.0) top = T(0);
+            else top = lds_interp(G, ntop, DNV);
+        }
+        if (lane == 63) Qn = top;
     }
-    return (lds_interp(G, nhi, DNV) - lds_interp(G, nlo, DNV)) * sgn;
+    return (Qn - Q) * sgn;
 }
 
 // Column record in walk order (increasing row coordinate a): slopes of the
@@ -274,9 +284,14 @@ __device__ __forceinline__ void back_walk_accum(const DevGeom& g, const CTLAB_GM
                                                 const CTLAB_GMEM int* __restrict__ views, int nviews, int NR, int j,
                                                 int rt, const CTLAB_GMEM ColRec* __restrict__ rec,
                                                 const CTLAB_GMEM unsigned short* __restrict__ start,
-                                                int SJ, int SR, int lane, T* G, T (&acc)[NM])
+                                                int SJ, int SR, int lane, T* G, T* A, T (&acc)[NM])
 {
     if (lane == 0) G[0] = T(0);
+    // row accumulators live in LDS (A[i*64 + lane]): the row index varies at run
+    // time, and indexing registers with it costs four scalar-set / move pairs
+    // per FP64 update
+#pragma unroll
+    for (int i = 0; i < NM; ++i) A[i * 64 + lane] = T(0);
     const int r0 = rt * NM;
     const int nm = min(NM, NR - r0);
     const int ZN = g.ZN, DNU = g.DNU, DNV = g.DNV;
@@ -284,9 +299,6 @@ __device__ __forceinline__ void back_walk_accum(const DevGeom& g, const CTLAB_GM
     const double inv_dx = 1.0 / g.dx;
     const long srow = (long)j * SR + (r0 >> 3);
     const T sgn = g.dzh < 0.0 ? T(-1) : T(1);
-
-#pragma unroll
-    for (int i = 0; i < NM; ++i) acc[i] = T(0);
 
     int pn = nviews > 0 ? views[0] : 0;
     int stn = nviews > 0 ? start[(long)pn * SJ * SR + srow] : 0xFFFF;
@@ -303,42 +315,50 @@ __device__ __forceinline__ void back_walk_accum(const DevGeom& g, const CTLAB_GM
         const double rs = sj * inv_dx;
         const double inv_sj = 1.0 / sj;
         const double inv_rs = fabs(g.dx * inv_sj);
-        const int asc = vp.asc;
-        const CTLAB_GMEM ColRec* __restrict__ R = rec + (long)p * DNU;
-        const CTLAB_GMEM T* __restrict__ Pv = P + (long)p * DNU * DNV + vl;
+        // walk-order column c: its record rp[0] and v-prefix row pp[0]; the next
+        // column's (rq, pq) are loaded one column ahead; pointers step instead of
+        // being recomputed (the scalar unit is as busy as the vector unit here)
         int c = st;
-        ColRec r = R[c];
+        const CTLAB_GMEM ColRec* __restrict__ rq = rec + (long)p * DNU + c;
+        const long pstep = vp.asc ? DNV : -DNV;
+        const CTLAB_GMEM T* __restrict__ pq = P + (long)p * DNU * DNV + vl + (long)(vp.asc ? c : DNU - 1 - c) * DNV;
+        ColRec r = rq[0];
         double alo = vp.rbase + rs * r.klo;
         if (alo >= (double)(r0 + nm)) continue;
         double ahi = vp.rbase + rs * r.khi;
         double iw = inv_rs * r.invdk;
-        const int c1 = min(c + 1, DNU - 1);
-        T Pr0 = Pv[(long)(asc ? c : DNU - 1 - c) * DNV];
-        ColRec rn = R[c1];
-        T Pr1 = Pv[(long)(asc ? c1 : DNU - 1 - c1) * DNV];
-        T D = column_delta(g, G, Pr0, r.esdz, inv_sj, vp.beta0, sgn, lane);
+        // first integer row boundary at or above the column's upper edge
+        int mend = __builtin_amdgcn_readfirstlane((int)ceil(fmin(fmax(ahi, -1.0e9), 1.0e9)));
+        T Pr0 = pq[0];
+        if (c + 1 < DNU) { rq += 1; pq += pstep; }
+        ColRec rn = rq[0];
+        T Pr1 = pq[0];
+        T D = column_delta(g, G, Pr0, r.esdz, inv_sj, vp.beta0, vp.top_total, sgn, lane);
         T Tsum = T(0), Wprev = T(0);
-        // not unrolled (each copy would carry its own column walk); the row
-        // index is uniform, so acc[] is indexed through s_set_gpr_idx
+        // not unrolled (each copy would carry its own column walk)
         for (int i = 0; i <= nm; ++i) {
-            const double mp = (double)(r0 + i);
-            while (ahi <= mp && c < DNU - 1) {
+            const int mi = r0 + i;
+            const double mp = (double)mi;
+            while (mi >= mend && c < DNU - 1) {
                 Tsum += D;
                 ++c;
                 alo = ahi;
                 ahi = vp.rbase + rs * rn.khi;
+                mend = __builtin_amdgcn_readfirstlane((int)ceil(fmin(fmax(ahi, -1.0e9), 1.0e9)));
                 iw = inv_rs * rn.invdk;
-                D = column_delta(g, G, Pr1, rn.esdz, inv_sj, vp.beta0, sgn, lane);
-                const int cn = min(c + 1, DNU - 1);
-                rn = R[cn];
-                Pr1 = Pv[(long)(asc ? cn : DNU - 1 - cn) * DNV];
+                D = column_delta(g, G, Pr1, rn.esdz, inv_sj, vp.beta0, vp.top_total, sgn, lane);
+                if (c + 1 < DNU) { rq += 1; pq += pstep; }
+                rn = rq[0];
+                Pr1 = pq[0];
             }
             const double fr = fmin(fmax((mp - alo) * iw, 0.0), 1.0);
             const T W = Tsum + (T)fr * D;
-            if (i > 0) acc[__builtin_amdgcn_readfirstlane(i - 1)] += W - Wprev;
+            if (i > 0) A[(i - 1) * 64 + lane] += W - Wprev;
             Wprev = W;
         }
     }
+#pragma unroll
+    for (int i = 0; i < NM; ++i) acc[i] = A[i * 64 + lane];
 }
 
 // Writes one backprojected tile (lane = z): for voxel row i of the tile,
@@ -369,10 +389,10 @@ __device__ __forceinline__ void back_walk_wave(const DevGeom& g, const CTLAB_GME
                                                int nviews, int yslab, int accumulate, int NR, int j, int rt,
                                                const CTLAB_GMEM ColRec* __restrict__ rec,
                                                const CTLAB_GMEM unsigned short* __restrict__ start, int SJ, int SR, int lane,
-                                               T* G)
+                                               T* G, T* A)
 {
     T acc[NM];
-    back_walk_accum<T, NM>(g, P, views, nviews, NR, j, rt, rec, start, SJ, SR, lane, G, acc);
+    back_walk_accum<T, NM>(g, P, views, nviews, NR, j, rt, rec, start, SJ, SR, lane, G, A, acc);
     back_store_tile<T, NM>(g, acc, yslab, NR, j, rt, lane, [&](long idx, float v) {
         vol[idx] = accumulate ? vol[idx] + v : v;
     });

@@ -104,12 +104,13 @@ dd_back_walk(DevGeom g, const T* __restrict__ P, float* __restrict__ vol,
              const unsigned short* __restrict__ start, int SJ, int SR)
 {
     __shared__ T sG[4][72];
+    __shared__ T sA[4][NM * 64];
     const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int rt = blockIdx.x % ntr;
     const int j = (blockIdx.x / ntr) * 4 + wv;
     if (j >= NS) return;
     back_walk_wave<T, NM>(g, gmem(P), gmem(vol), gmem(views), nviews, yslab, accumulate, NR, j, rt, gmem(rec),
-                          gmem(start), SJ, SR, threadIdx.x & 63, sG[wv]);
+                          gmem(start), SJ, SR, threadIdx.x & 63, sG[wv], sA[wv]);
 }
 
 // ---------------------------------------------------------------------------

@@ -215,6 +215,21 @@ static int build_tables(ctlab_plan* P)
                 g.cone ? g.dx * g.dx / std::fabs(dk) : g.dx * g.dx * std::fabs(t) / std::fabs(dk);
         }
         V.asc = (V.s0 * dk0) > 0;
+        // row coordinate of the top voxel boundary, (ZN - beta0) * esdz / sj - zhat0 / dzh, is
+        // bilinear in (esdz, 1/sj): its corners bound it over all slabs and columns
+        V.top_total = 0;
+        if (g.cone) {
+            double emin = P->esdz[(size_t)p * DNU], emax = emin;
+            for (int u = 1; u < DNU; ++u) {
+                emin = std::min(emin, P->esdz[(size_t)p * DNU + u]);
+                emax = std::max(emax, P->esdz[(size_t)p * DNU + u]);
+            }
+            double lo = HUGE_VAL;
+            for (double e : {emin, emax})
+                for (double sj : {V.s0, slast})
+                    lo = std::min(lo, (g.ZN - V.beta0) * e / sj - P->zhat0 / P->dzh);
+            V.top_total = lo >= (double)DNV;
+        }
         (yslab ? P->views_y : P->views_x).push_back(p);
         // walk-order (increasing row coordinate) column records for the backprojector
         for (int c = 0; c < DNU; ++c) {
