@@ -252,6 +252,82 @@ def batch_bench(args):
     print(json.dumps(res))
 
 
+def cfg3_bench(args):
+    """cfg3: OS-SART (10 subsets, 20 iterations) and FISTA-TV (100 iterations, lambda 1e-4)
+    on the cfg1 fan-beam geometry (256x256, 360 views, 888-cell arc detector), data = the
+    projection of the 10-ellipse pixel phantom (ellipses reach beyond the image, so exact
+    line integrals would hold mass the image cannot represent) with Poisson noise at
+    I0 = 1e5 and log transform; GPU results against the FP64 oracle drivers and the phantom."""
+    import torch
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import recon_ref as RR
+    from ctlab_amd import dd
+    from ctlab_amd import geometry as G
+    from ctlab_amd import phantom as PH
+    from ctlab_amd import recon as RC
+    dev = torch.device("cuda:0")
+    g = G.fan_multislice(XN=256, YN=256, SLN=1, dx=1.0, PN=360, DNU=888, col_size=1.0239, sid=541.0, sdd=949.0)
+    ells = PH.random_ellipses(1, 10, 256, 256, g.dx)
+    ph = PH.voxelize(ells, 256, 256, 1, g.dx, 1.0)
+    plan = dd.Plan(g)
+    clean = plan.proj(torch.from_numpy(np.ascontiguousarray(ph)).to(dev)).cpu().numpy()
+    b = PH.add_poisson_noise(clean, 1e5, 3).reshape(-1)
+    bt = torch.from_numpy(b).to(dev)
+
+    def psnr(x):
+        x = np.asarray(x, np.float64).reshape(-1)
+        ref = ph.reshape(-1).astype(np.float64)
+        return 10 * np.log10(ref.max() ** 2 / np.mean((x - ref) ** 2))
+
+    def rel(a, r):
+        a, r = np.asarray(a, np.float64).reshape(-1), np.asarray(r, np.float64).reshape(-1)
+        return float(np.linalg.norm(a - r) / np.linalg.norm(r))
+
+    out = {"metric": METRIC, "mode": "cfg3 OS-SART and FISTA-TV on the cfg1 fan-beam geometry",
+           "config": {"workload": "cfg3: 256x256 fan beam, 360 views, 888-cell arc detector (1.0239 mm), SID 541, "
+                                  "SDD 949, 10 random ellipses, Poisson I0 = 1e5 + log",
+                      "pixel_mm": g.dx},
+           "data": "projection of the pixel phantom by the library's projector, Poisson noise I0 = 1e5, log "
+                   "(same model for data and reconstruction)",
+           "max_line_integral": float(clean.max())}
+    # OS-SART, 10 subsets, 20 iterations
+    rc = RC.Recon(plan, 10)
+    rc.set_data(bt)
+    x = torch.zeros(g.vol_shape, device=dev)
+    rc.os_sart(x, 1)                      # graph capture + warm-up
+    x.zero_()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    rc.os_sart(x, 20)
+    torch.cuda.synchronize()
+    t_sart = time.perf_counter() - t0
+    xs = x.cpu().numpy()
+    t0 = time.perf_counter()
+    xs_ref = RR.os_sart(g, b, 10, 20)
+    t_sart_ref = time.perf_counter() - t0
+    out["os_sart"] = {"subsets": 10, "iterations": 20, "gpu_s": t_sart, "oracle_cpu_s": t_sart_ref,
+                      "psnr_gpu": psnr(xs), "psnr_oracle": psnr(xs_ref), "rel_l2_vs_fp64_oracle": rel(xs, xs_ref)}
+    # FISTA-TV, 100 iterations, lambda 1e-4 (FGP prox, 10 inner steps)
+    rf = RC.Recon(plan, 1)
+    rf.set_data(bt)
+    L = float(np.float32(1.01 * rf.lipschitz(30)))
+    xf = torch.zeros(g.vol_shape, device=dev)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    obj = rf.fista_tv(xf, 100, 1e-4, 10, L=L)
+    torch.cuda.synchronize()
+    t_fista = time.perf_counter() - t0
+    xfn = xf.cpu().numpy()
+    t0 = time.perf_counter()
+    xf_ref, obj_ref = RR.fista_tv(g, b, 100, 1e-4, 10, L)
+    t_fista_ref = time.perf_counter() - t0
+    out["fista_tv"] = {"iterations": 100, "tv_lambda": 1e-4, "L": L, "gpu_s": t_fista, "oracle_cpu_s": t_fista_ref,
+                       "psnr_gpu": psnr(xfn), "psnr_oracle": psnr(xf_ref), "rel_l2_vs_fp64_oracle": rel(xfn, xf_ref),
+                       "objective_rel_diff_final": abs(obj[-1] - obj_ref[-1]) / obj_ref[-1],
+                       "objective_first_last": [float(obj[0]), float(obj[-1])]}
+    print(json.dumps(out))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -261,8 +337,10 @@ def main():
     ap.add_argument("--cpu-views", type=int, default=16)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-tune", action="store_true")
+    ap.add_argument("--tiles", default="", help="fixed projector,backprojector tiles (e.g. 8,8) instead of tuning")
     ap.add_argument("--learned", action="store_true", help="cfg5: learned FISTA vs plain FISTA-TV")
     ap.add_argument("--batch", action="store_true", help="cfg4: batched OS-SART over uneven problems")
+    ap.add_argument("--cfg3", action="store_true", help="cfg3: OS-SART and FISTA-TV on the 2-D fan geometry")
     ap.add_argument("--problems", type=int, default=32)
     ap.add_argument("--subsets", type=int, default=10)
     ap.add_argument("--samples", type=int, default=2000)
@@ -280,6 +358,8 @@ def main():
         return learned_bench(args)
     if args.batch:
         return batch_bench(args)
+    if args.cfg3:
+        return cfg3_bench(args)
 
     import torch
     from ctlab_amd import dd
@@ -301,7 +381,9 @@ def main():
     stream = torch.cuda.current_stream()
     # tile autotuner on scratch copies (setup, not timed)
     tune = None
-    if not args.no_tune:
+    if args.tiles:
+        plan.set_tiles(*[int(t) for t in args.tiles.split(",")])
+    elif not args.no_tune:
         tune = plan.autotune(vol.clone(), torch.empty_like(prj))
         torch.cuda.synchronize()
 
@@ -352,7 +434,8 @@ def main():
         "autotune_ms": tune,
         "roofline": {"bound": "hbm", "achieved": ach / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
                      "frac": ach / HBM_PEAK,
-                     "traffic": (traffic or {}).get(dominant + "_bytes_per_call"),
+                     "traffic": ((traffic or {}).get("by_tiles", {}).get("%d,%d" % plan.tiles(), {})
+                                 .get(dominant + "_bytes_per_call")),
                      "kernel": "dd_proj_walk (+zprefix)" if dominant == "proj"
                                else "dd_back_walk x2 (+prjprefix)",
                      "pair_frac": (2 * bytes_op / t_pair) / HBM_PEAK},
