@@ -137,6 +137,7 @@ __device__ __forceinline__ void proj_walk_wave(const DevGeom& g, const CTLAB_GME
         const double a_n = __shfl_down(a_l, 1, 64);
         const unsigned long long act = __ballot(lane < nu && (vp.asc ? a_n > a_l : a_l > a_n));
         const T qj = (T)(g.cone ? 1.0 / (sj * sj) : 1.0 / fabs(sj));
+        const double t_l = g.cone ? sj * tau_l : 1.0;     // z magnification of column `lane`
         const CTLAB_GMEM T* __restrict__ base = cz + j * slab_off + zl;
         int m = __builtin_amdgcn_readlane(m_l, b0);
         // rows m, m+1, m+2 in flight (row indices clamped to the last row, as
@@ -168,7 +169,7 @@ __device__ __forceinline__ void proj_walk_wave(const DevGeom& g, const CTLAB_GME
             Eprev = E;
             const int uq = vp.asc ? qc - 1 : nu - qc;      // tile column crossed at this step
             if (q <= nu && ((act >> uq) & 1ull)) {
-                const double t = g.cone ? sj * readlane_d(tau_l, uq) : 1.0;
+                const double t = readlane_d(t_l, uq);
                 T* Gq = G + (q - 1) * kGStride;
                 Gq[lane + 1] = H;
                 const T Db = lds_interp(Gq, vp.beta0 + t * zb, ZN);
@@ -245,24 +246,24 @@ __device__ __forceinline__ void prjprefix_wave(const DevGeom& g, const CTLAB_GME
 // where the prefix is the column total (above) or 0 (below), and is
 // interpolated only when it lands on the detector.  top_total: the view's top
 // boundary is above the detector for every slab and column (decided on the host).
+// lzb = (lane - beta0) / sj (per view and slab), zoff = zhat0 / dzh: the lower
+// boundary of voxel z = lane maps to the fractional detector row esdz * lzb - zoff.
 template <typename T>
-__device__ __forceinline__ T column_delta(const DevGeom& g, T* G, T Pu, double esdz, double inv_sj,
-                                          double beta0, int top_total, T sgn, int lane)
+__device__ __forceinline__ T column_delta(const DevGeom& g, T* G, T Pu, double esdz, double lzb, double zoff,
+                                          double inv_sj, double beta0, int top_total, T sgn, int lane)
 {
     const int DNV = g.DNV;
     G[lane + 1] = Pu;
     if (!g.cone)   // multi-slice fan beam: boundaries are the integer rows
         return lds_interp(G, (double)(lane + 1), DNV) - lds_interp(G, (double)lane, DNV);
-    const double inv = inv_sj * esdz;   // 1 / (t * dzh)
-    const double off = -beta0 * inv - g.zhat0 / g.dzh;
-    const T Q = lds_interp(G, (double)lane * inv + off, DNV);
+    const T Q = lds_interp(G, esdz * lzb - zoff, DNV);
     T Qn = shift_down1(Q);
     if (g.ZN == 64) {
         T top;
         if (top_total) {
             top = readlane_t(Pu, DNV - 1);
         } else {
-            const double ntop = 64.0 * inv + off;
+            const double ntop = (64.0 - beta0) * inv_sj * esdz - zoff;
             if (ntop >= (double)DNV) top = readlane_t(Pu, DNV - 1);
             else if (ntop <= 0.0) top = T(0);
             else top = lds_interp(G, ntop, DNV);
@@ -299,6 +300,7 @@ __device__ __forceinline__ void back_walk_accum(const DevGeom& g, const CTLAB_GM
     const double inv_dx = 1.0 / g.dx;
     const long srow = (long)j * SR + (r0 >> 3);
     const T sgn = g.dzh < 0.0 ? T(-1) : T(1);
+    const double zoff = g.zhat0 / g.dzh;
 
     int pn = nviews > 0 ? views[0] : 0;
     int stn = nviews > 0 ? start[(long)pn * SJ * SR + srow] : 0xFFFF;
@@ -315,6 +317,7 @@ __device__ __forceinline__ void back_walk_accum(const DevGeom& g, const CTLAB_GM
         const double rs = sj * inv_dx;
         const double inv_sj = 1.0 / sj;
         const double inv_rs = fabs(g.dx * inv_sj);
+        const double lzb = ((double)lane - vp.beta0) * inv_sj;
         // walk-order column c: its record rp[0] and v-prefix row pp[0]; the next
         // column's (rq, pq) are loaded one column ahead; pointers step instead of
         // being recomputed (the scalar unit is as busy as the vector unit here)
@@ -333,7 +336,7 @@ __device__ __forceinline__ void back_walk_accum(const DevGeom& g, const CTLAB_GM
         if (c + 1 < DNU) { rq += 1; pq += pstep; }
         ColRec rn = rq[0];
         T Pr1 = pq[0];
-        T D = column_delta(g, G, Pr0, r.esdz, inv_sj, vp.beta0, vp.top_total, sgn, lane);
+        T D = column_delta(g, G, Pr0, r.esdz, lzb, zoff, inv_sj, vp.beta0, vp.top_total, sgn, lane);
         T Tsum = T(0), Wprev = T(0);
         // not unrolled (each copy would carry its own column walk)
         for (int i = 0; i <= nm; ++i) {
@@ -346,7 +349,7 @@ __device__ __forceinline__ void back_walk_accum(const DevGeom& g, const CTLAB_GM
                 ahi = vp.rbase + rs * rn.khi;
                 mend = __builtin_amdgcn_readfirstlane((int)ceil(fmin(fmax(ahi, -1.0e9), 1.0e9)));
                 iw = inv_rs * rn.invdk;
-                D = column_delta(g, G, Pr1, rn.esdz, inv_sj, vp.beta0, vp.top_total, sgn, lane);
+                D = column_delta(g, G, Pr1, rn.esdz, lzb, zoff, inv_sj, vp.beta0, vp.top_total, sgn, lane);
                 if (c + 1 < DNU) { rq += 1; pq += pstep; }
                 rn = rq[0];
                 Pr1 = pq[0];
