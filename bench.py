@@ -97,56 +97,92 @@ def learned_bench(args):
         log(f"lambda {lam}: best validation PSNR {sweep[lam]:.2f} dB")
     lam = max(sweep, key=sweep.get)
     per = 4
-    xs, ts, _, t_gen = LE.generate_pairs(args.samples // per, per, tshape, 120, 1e4, 1000, dev, lam,
-                                         kmax=min(args.iters, 60), log=log)
-    loss_id = LE.identity_loss(xs, ts)
-    log(f"identity-map loss {loss_id:.4e}")
-    params, losses, t_train, steps = LE.train(xs, ts, tshape, args.epochs, lr=1e-3, log=log)
-    del xs, ts
-    np.save(os.path.join(ROOT, "gpurun_out", "learned_prox_params.npy"), params) if os.path.isdir(
-        os.path.join(ROOT, "gpurun_out")) else None
+    if args.params:
+        params = np.load(args.params).astype(np.float32)
+        losses, t_gen, t_train, steps, loss_id = [], 0.0, 0.0, 0, None
+        log(f"parameters from {args.params}: generation and training skipped")
+    else:
+        xs, ts, _, t_gen = LE.generate_pairs(args.samples // per, per, tshape, 120, 1e4, 1000, dev, lam,
+                                             kmin=5, kmax=args.iters, log=log)
+        loss_id = LE.identity_loss(xs, ts)
+        log(f"identity-map loss {loss_id:.4e}")
+        params, losses, t_train, steps = LE.train(xs, ts, tshape, args.epochs, lr=1e-3, log=log)
+        del xs, ts
+        np.save(os.path.join(ROOT, "gpurun_out", "learned_prox_params.npy"), params) if os.path.isdir(
+            os.path.join(ROOT, "gpurun_out")) else None
     # validation: the trained step inside FISTA on the validation phantom
     net_v = LE.ScaledCNN(gv.vol_shape, params)
     _, hvl, _ = LE.fista_tv_run(rv, Lt, gv.vol_shape, args.iters, lam, 10, ph_v, zv, dev, learned=net_v)
+    _, hvo, _ = LE.fista_tv_run(rv, Lt, gv.vol_shape, args.iters, lam, 10, ph_v, zv, dev, learned=net_v,
+                                learned_every=5, feedback=False)
     del net_v
     # evaluation phantom, 512x512x64, seed 7
     ge = LE.cone_geometry(512, 512, 64, 120)
     pe = dd.Plan(ge)
     re_ = RC.Recon(pe, 1)
-    ph = LE.make_problem(ge, pe, re_, 7, 1e4, dev)
+    ph, noise = LE.make_problem(ge, pe, re_, 7, 1e4, dev, with_noise=True)
+    f_truth = re_.data_fidelity(ph)
     Le = re_.lipschitz(30) * 1.01
     ze = LE.covered_slices(ge)
     _, h_plain, _ = LE.fista_tv_run(re_, Le, ge.vol_shape, args.iters, lam, 10, ph, ze, dev)
     net = LE.ScaledCNN(ge.vol_shape, params)
     _, h_learn, _ = LE.fista_tv_run(re_, Le, ge.vol_shape, args.iters, lam, 10, ph, ze, dev, learned=net)
+    _, h_every, _ = LE.fista_tv_run(re_, Le, ge.vol_shape, args.iters, lam, 10, ph, ze, dev, learned=net,
+                                    learned_every=5)
+    _, h_out, _ = LE.fista_tv_run(re_, Le, ge.vol_shape, args.iters, lam, 10, ph, ze, dev, learned=net,
+                                  learned_every=5, feedback=False)
+    _, h_disc, _ = LE.fista_tv_run(re_, Le, ge.vol_shape, args.iters, lam, 10, ph, ze, dev, learned=net,
+                                   learned_every=5, feedback=False, noise_level=noise)
     target = 40.0
     best_plain = max(h["psnr"] for h in h_plain)
     tp = LE.time_to_psnr(h_plain, target)
-    tl = LE.time_to_psnr(h_learn, target)
     # if plain never reaches 40 dB, compare at the best PSNR plain FISTA-TV reaches
     ref_db = target if tp[0] is not None else best_plain
-    tp2, tl2 = LE.time_to_psnr(h_plain, ref_db), LE.time_to_psnr(h_learn, ref_db)
+    tp2 = LE.time_to_psnr(h_plain, ref_db)
     rate, t_fwd = LE.cnn_forward_rate(ge.vol_shape, params, dev)
+
+    def summary(h):
+        t40 = LE.time_to_psnr(h, target)
+        tr = LE.time_to_psnr(h, ref_db)
+        out = {"best_psnr": max(x["psnr"] for x in h), "final_psnr": h[-1]["psnr"], "t_total_s": h[-1]["t"],
+               "t_to_40dB_s": t40[0], "t_to_ref_s": tr[0], "iters_to_ref": tr[1], "fidelity_at_ref": tr[2],
+               "accepted_steps": sum(x["learned"] for x in h),
+               "speedup_to_ref_vs_plain": (tp2[0] / tr[0]) if (tp2[0] and tr[0]) else None}
+        # at matched data residual: every accepted learned output against the first plain
+        # iterate whose ||Ax-b||^2 is as low
+        matched = []
+        for x in h:
+            if x["learned"]:
+                (tp_, ip_, pp_), _ = LE.at_matched_fidelity(h_plain, h, x["fidelity"])
+                matched.append({"iter": x["iter"], "t": x["t"], "psnr": x["psnr"], "fidelity": x["fidelity"],
+                                "plain_iter": ip_, "plain_t": tp_, "plain_psnr": pp_})
+        out["at_matched_fidelity"] = matched
+        return out
+
     res = {
         "metric": METRIC, "mode": "cfg5 learned safeguarded FISTA vs plain FISTA-TV",
         "train": {"pairs": (args.samples // per) * per, "phantoms": args.samples // per, "snapshots_per_phantom": per,
                   "shape": list(tshape), "views": 120, "I0": 1e4, "epochs": args.epochs,
                   "steps": steps, "gen_s": t_gen, "train_s": t_train, "loss_per_epoch": losses,
-                  "loss_identity_map": loss_id},
-        "validation_256": {"plain_best_psnr": sweep[lam], "learned_best_psnr": max(h["psnr"] for h in hvl),
-                           "learned_accepted_steps": sum(h["learned"] for h in hvl)},
+                  "loss_identity_map": loss_id, "params_loaded": bool(args.params)},
+        "validation_256": {"plain_best_psnr": sweep[lam], "learned_in_loop_best_psnr": max(h["psnr"] for h in hvl),
+                           "learned_in_loop_accepted": sum(h["learned"] for h in hvl),
+                           "learned_output_best_psnr": max(h["psnr"] for h in hvo),
+                           "learned_output_accepted": sum(h["learned"] for h in hvo)},
         "tv_lambda_sweep_val_psnr": {str(k): v for k, v in sweep.items()}, "tv_lambda": lam,
-        "eval": {"shape": [512, 512, 64], "views": 120, "I0": 1e4, "iters": args.iters, "psnr_slices": [ze.start, ze.stop]},
-        "plain": {"best_psnr": best_plain, "final_psnr": h_plain[-1]["psnr"], "t_total_s": h_plain[-1]["t"],
-                  "t_to_ref_s": tp2[0], "iters_to_ref": tp2[1], "fidelity_at_ref": tp2[2]},
-        "learned": {"best_psnr": max(h["psnr"] for h in h_learn), "final_psnr": h_learn[-1]["psnr"],
-                    "t_total_s": h_learn[-1]["t"], "t_to_ref_s": tl2[0], "iters_to_ref": tl2[1],
-                    "fidelity_at_ref": tl2[2], "accepted_steps": sum(h["learned"] for h in h_learn)},
-        "ref_psnr_db": ref_db, "reached_40dB": {"plain": tp[0] is not None, "learned": tl[0] is not None},
-        "speedup_to_ref": (tp2[0] / tl2[0]) if (tp2[0] and tl2[0]) else None,
+        "eval": {"shape": [512, 512, 64], "views": 120, "I0": 1e4, "iters": args.iters,
+                 "psnr_slices": [ze.start, ze.stop], "fidelity_of_phantom": f_truth,
+                 "noise_energy_estimate": noise},
+        "plain": summary(h_plain),
+        "learned_in_loop": summary(h_learn),
+        "learned_in_loop_every5": summary(h_every),
+        "learned_output_every5": summary(h_out),
+        "learned_output_every5_discrepancy": summary(h_disc),
+        "ref_psnr_db": ref_db,
         "cnn_forward": {"shape": [512, 512, 64], "s": t_fwd, "tflops": rate / 1e12,
                         "mfma_util_vs_bf16_dense_peak": rate / 2.5e15},
-        "history": {"plain": h_plain, "learned": h_learn},
+        "history": {"plain": h_plain, "learned_in_loop": h_learn, "learned_in_loop_every5": h_every,
+                    "learned_output_every5": h_out, "learned_output_every5_discrepancy": h_disc},
     }
     print(json.dumps(res))
 
@@ -346,6 +382,7 @@ def main():
     ap.add_argument("--samples", type=int, default=2000)
     ap.add_argument("--epochs", type=int, default=3)
     ap.add_argument("--iters", type=int, default=60)
+    ap.add_argument("--params", default="", help="--learned: trained parameters (.npy) instead of training")
     ap.add_argument("--calib-copy-mb", type=int, default=0,
                     help="first run a plain device-to-device copy of this many MiB (profiler calibration)")
     args = ap.parse_args()

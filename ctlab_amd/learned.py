@@ -5,7 +5,20 @@
     x_cnn   = x_tv + net(x_tv)                               (learned correction, BF16 MFMA)
     x_{k+1} = x_cnn   if ||A x_cnn - b||^2 < ||A x_k - b||^2 (safeguard on the exact data fidelity)
               x_tv    otherwise
-    y_{k+1} = FISTA momentum (src/FastIterativeShrinkageAlgorithm.h:17-42)
+    y_{k+1} = FISTA momentum (src/FastIterativeShrinkageAlgorithm.h:17-42); after an
+              accepted learned step the momentum restarts (t = 1, x_{k} := x_{k+1}), so
+              the extrapolation does not amplify the network's correction
+
+With learned_every = N > 1 the learned step is tried every N-th iteration
+only (the others are plain FISTA-TV steps), so the network sees iterates close
+to the plain FISTA-TV ones it was trained on.  Output mode (feedback = False):
+the iterates are exactly those of plain FISTA-TV, and every N iterations the
+output is x_cnn = x_k + net(x_k) when ||A x_cnn - b||^2 <= ||A x_k - b||^2, else
+x_k; the correction never feeds back into the iteration.  With noise_level
+given (the discrepancy principle, a variant of the safeguard), a learned output
+is also accepted when ||A x_cnn - b||^2 <= noise_level: it fits the data to
+within the noise.  Near convergence the plain iterates fit the noise, so a
+strict decrease rejects every step towards the truth.
 
 The learned part is a residual on top of the TV prox: a network that has
 learned nothing (net = 0) gives back plain FISTA-TV exactly.  Training pairs
@@ -103,8 +116,16 @@ class ScaledCNN:
 EVAL_POPULATION = (512, 512, 64)   # ellipsoid population box (XN, YN, ZN) of the evaluation phantom
 
 
-def make_problem(g, plan, rc, seed, I0, dev, population=EVAL_POPULATION):
-    """Seeded phantom on the device and its noisy data set on rc; returns the phantom.
+def noise_energy(b, I0):
+    """Expected ||noise||^2 of log-transformed Poisson data, from the data alone:
+    var(-log(N/I0)) ~ 1/E[N] = exp(b)/I0 per ray."""
+    import torch
+    return float((torch.exp(b.double()) / I0).sum())
+
+
+def make_problem(g, plan, rc, seed, I0, dev, population=EVAL_POPULATION, with_noise=False):
+    """Seeded phantom on the device and its noisy data set on rc; returns the
+    phantom (and the data's expected noise energy when with_noise).
 
     The phantom is a window of g's size cut at a seeded random position from 20
     random ellipsoids spread over a population box (voxel counts, same voxel
@@ -120,20 +141,21 @@ def make_problem(g, plan, rc, seed, I0, dev, population=EVAL_POPULATION):
     ells[:, 1] -= off[1]
     ells[:, 2] -= off[2]
     ph = voxelize_torch(ells, g.vol_shape, g.dx, g.dz, dev)
-    rc.set_data(noisy_projection(plan, ph, I0, seed))
-    return ph
+    b = noisy_projection(plan, ph, I0, seed)
+    rc.set_data(b)
+    return (ph, noise_energy(b, I0)) if with_noise else ph
 
 
-def snapshot_iters(rng, kmax, n):
-    """n distinct FISTA iterations in [1, kmax], denser early (where the iterates change most)."""
+def snapshot_iters(rng, kmin, kmax, n):
+    """n distinct FISTA iterations in [kmin, kmax], log-uniform (denser early)."""
     ks = set()
     while len(ks) < n:
-        ks.add(int(np.clip(round(np.exp(rng.uniform(0.0, np.log(kmax)))), 1, kmax)))
+        ks.add(int(np.clip(round(np.exp(rng.uniform(np.log(kmin), np.log(kmax)))), kmin, kmax)))
     return sorted(ks)
 
 
-def generate_pairs(n_phantoms, per_phantom, shape, views, I0, seed0, dev, tv_lambda, tv_iters=10, kmax=60,
-                   log=None):
+def generate_pairs(n_phantoms, per_phantom, shape, views, I0, seed0, dev, tv_lambda, tv_iters=10, kmin=5,
+                   kmax=100, log=None, population=EVAL_POPULATION):
     """n_phantoms * per_phantom (x_k, phantom) pairs on the device: x_k is the
     FISTA-TV (tv_lambda) iterate at snapshot iterations k.  Returns pairs, L, seconds."""
     import torch
@@ -146,8 +168,8 @@ def generate_pairs(n_phantoms, per_phantom, shape, views, I0, seed0, dev, tv_lam
     xs, ts = [], []
     t0 = time.perf_counter()
     for i in range(n_phantoms):
-        ph = make_problem(g, plan, rc, seed0 + i, I0, dev)
-        ks = snapshot_iters(rng, kmax, per_phantom)
+        ph = make_problem(g, plan, rc, seed0 + i, I0, dev, population)
+        ks = snapshot_iters(rng, kmin, kmax, per_phantom)
         _, _, snaps = fista_tv_run(rc, L, g.vol_shape, ks[-1], tv_lambda, tv_iters, None, None, dev, snaps=ks)
         for k in ks:
             xs.append(snaps[k])
@@ -163,6 +185,19 @@ def identity_loss(xs, ts):
     """The training loss of net = 0: 0.5*mean((x - phantom)^2) in the scaled units."""
     s2 = INPUT_SCALE * INPUT_SCALE
     return float(np.mean([0.5 * s2 * float(((x - t) ** 2).mean()) for x, t in zip(xs, ts)]))
+
+
+def network_loss(xs, ts, shape, params):
+    """The same loss with the trained correction applied (forward only)."""
+    import torch
+    net = ScaledCNN(shape, params)
+    out = torch.empty_like(xs[0])
+    s2 = INPUT_SCALE * INPUT_SCALE
+    vals = []
+    for x, t in zip(xs, ts):
+        net(x, out)
+        vals.append(0.5 * s2 * float(((out - t) ** 2).mean()))
+    return float(np.mean(vals))
 
 
 def train(xs, ts, shape, epochs, lr=1e-3, seed=0, log=None):
@@ -200,11 +235,14 @@ def _timed(fn):
     return r, time.perf_counter() - t0
 
 
-def fista_tv_run(rc, L, b_shape_vol, iters, tv_lambda, tv_iters, truth, zsl, dev, learned=None, snaps=None):
-    """Plain FISTA-TV (learned=None) or the safeguarded learned FISTA, step by
-    step with the library's kernels.  Returns x, the per-iteration history (wall
-    time of the GPU work, synchronised; PSNR when truth is given; data fidelity;
-    accepted learned step) and {k: copy of the iterate x_k} for k in snaps."""
+def fista_tv_run(rc, L, b_shape_vol, iters, tv_lambda, tv_iters, truth, zsl, dev, learned=None, snaps=None,
+                 learned_every=1, feedback=True, noise_level=None):
+    """Plain FISTA-TV (learned=None), the safeguarded learned FISTA (learned
+    given; tried every learned_every iterations) or its output mode
+    (feedback=False), step by step with the library's kernels.  Returns x, the
+    per-iteration history (wall time of the GPU work, synchronised; PSNR of the
+    output and of a rejected learned candidate when truth is given; data
+    fidelity; accepted learned step) and {k: copy of the iterate x_k}."""
     import torch
     shape = b_shape_vol
     n = int(np.prod(shape))
@@ -218,6 +256,10 @@ def fista_tv_run(rc, L, b_shape_vol, iters, tv_lambda, tv_iters, truth, zsl, dev
     work = torch.empty(7 * n, device=dev)
     t = 1.0
     f_x = rc.data_fidelity(x)
+    in_loop = learned is not None and feedback
+    out = x                               # the output of the current iteration
+    f_out = f_x
+    cand_psnr = None
     hist = []
     taken = {}
     elapsed = 0.0
@@ -227,10 +269,22 @@ def fista_tv_run(rc, L, b_shape_vol, iters, tv_lambda, tv_iters, truth, zsl, dev
             rc.gradient(y, g)
             z.copy_(y)
             RC.axpby(-1.0 / L, g, 1.0, z)
+            nonlocal out, f_out, cand_psnr
             accepted = False
-            if learned is None:
+            cand_psnr = None
+            tried = learned is not None and ((k + 1) % learned_every == 0 or k + 1 == iters)
+            if not in_loop or not tried:
                 RC.tv_prox_into(z, x, tv_lambda / L, tv_iters, shape, work)
                 f_x = rc.data_fidelity(x)
+                out, f_out = x, f_x
+                if tried:
+                    learned(x, xc)
+                    f_c = rc.data_fidelity(xc)
+                    if f_c <= f_x or (noise_level is not None and f_c <= noise_level):
+                        out, f_out = xc, f_c
+                        accepted = True
+                    elif truth is not None:
+                        cand_psnr = psnr(xc, truth, zsl)
             else:
                 RC.tv_prox_into(z, xt, tv_lambda / L, tv_iters, shape, work)
                 learned(xt, xc)
@@ -240,8 +294,14 @@ def fista_tv_run(rc, L, b_shape_vol, iters, tv_lambda, tv_iters, truth, zsl, dev
                     f_x = f_c
                     accepted = True
                 else:
+                    if truth is not None:
+                        cand_psnr = psnr(xc, truth, zsl)
                     x.copy_(xt)
                     f_x = rc.data_fidelity(x)
+                out, f_out = x, f_x
+            if accepted and in_loop:   # momentum restart
+                t = 1.0
+                x_old.copy_(x)
             tn = 0.5 * (1.0 + math.sqrt(1.0 + 4.0 * t * t))
             y.copy_(x)
             RC.fista_momentum(y, x_old, t, tn)
@@ -252,9 +312,20 @@ def fista_tv_run(rc, L, b_shape_vol, iters, tv_lambda, tv_iters, truth, zsl, dev
         elapsed += dt
         if snaps is not None and (k + 1) in snaps:
             taken[k + 1] = x.clone()
-        hist.append({"iter": k + 1, "t": elapsed, "psnr": psnr(x, truth, zsl) if truth is not None else None,
-                     "fidelity": f_x, "learned": acc})
-    return x, hist, taken
+        hist.append({"iter": k + 1, "t": elapsed, "psnr": psnr(out, truth, zsl) if truth is not None else None,
+                     "fidelity": f_out, "learned": acc, "rejected_candidate_psnr": cand_psnr})
+    return out, hist, taken
+
+
+def at_matched_fidelity(h_ref, h, level):
+    """First iteration of each run whose output fidelity is <= level:
+    (time, iteration, PSNR) for h_ref and for h."""
+    def first(hh):
+        for x in hh:
+            if x["fidelity"] <= level:
+                return x["t"], x["iter"], x["psnr"]
+        return None, None, None
+    return first(h_ref), first(h)
 
 
 def time_to_psnr(hist, target):
