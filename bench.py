@@ -131,8 +131,9 @@ def learned_bench(args):
                                     learned_every=5)
     _, h_out, _ = LE.fista_tv_run(re_, Le, ge.vol_shape, args.iters, lam, 10, ph, ze, dev, learned=net,
                                   learned_every=5, feedback=False)
+    # discrepancy principle with the customary factor tau = 1.1 on the data-only noise estimate
     _, h_disc, _ = LE.fista_tv_run(re_, Le, ge.vol_shape, args.iters, lam, 10, ph, ze, dev, learned=net,
-                                   learned_every=5, feedback=False, noise_level=noise)
+                                   learned_every=5, feedback=False, noise_level=1.1 * noise)
     target = 40.0
     best_plain = max(h["psnr"] for h in h_plain)
     tp = LE.time_to_psnr(h_plain, target)

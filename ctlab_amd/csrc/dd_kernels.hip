@@ -96,21 +96,36 @@ back_start_kernel(DevGeom g, const ColRec* __restrict__ rec, unsigned short* __r
 // of one slab orientation; lane = z.  Per view: the start column comes from
 // the table above (prefetched one view ahead), then columns are consumed in
 // walk order with the next column's record and P row in flight.
+//
+// Both slab orientations run in ONE launch: the first nby blocks take the
+// y-slab views and write vol_y, the rest take the x-slab views and write vol_x
+// (a scratch volume); back_add then sums them.  One grid instead of two
+// dependent launches leaves one tail of partly filled CUs instead of two.
 template <typename T, int NM>
 __global__ void __launch_bounds__(256)
-dd_back_walk(DevGeom g, const T* __restrict__ P, float* __restrict__ vol,
-             const int* __restrict__ views, int nviews, int yslab, int accumulate,
-             int NS, int NR, int ntr, const ColRec* __restrict__ rec,
-             const unsigned short* __restrict__ start, int SJ, int SR)
+dd_back_walk(DevGeom g, const T* __restrict__ P, float* __restrict__ vol_y, float* __restrict__ vol_x,
+             const int* __restrict__ views_y, int ny, const int* __restrict__ views_x, int nx, int nby,
+             int ntr_y, int ntr_x, const ColRec* __restrict__ rec, const unsigned short* __restrict__ start,
+             int SJ, int SR)
 {
     __shared__ T sG[4][72];
     __shared__ T sA[4][NM * 64];
     const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    const int rt = blockIdx.x % ntr;
-    const int j = (blockIdx.x / ntr) * 4 + wv;
+    const int ys = blockIdx.x < (unsigned)nby;
+    const int b = ys ? blockIdx.x : blockIdx.x - nby;
+    const int NS = ys ? g.YN : g.XN, NR = ys ? g.XN : g.YN, ntr = ys ? ntr_y : ntr_x;
+    const int rt = b % ntr;
+    const int j = (b / ntr) * 4 + wv;
     if (j >= NS) return;
-    back_walk_wave<T, NM>(g, gmem(P), gmem(vol), gmem(views), nviews, yslab, accumulate, NR, j, rt, gmem(rec),
-                          gmem(start), SJ, SR, threadIdx.x & 63, sG[wv], sA[wv]);
+    back_walk_wave<T, NM>(g, gmem(P), gmem(ys ? vol_y : vol_x), gmem(ys ? views_y : views_x), ys ? ny : nx, ys, 0,
+                          NR, j, rt, gmem(rec), gmem(start), SJ, SR, threadIdx.x & 63, sG[wv], sA[wv]);
+}
+
+// vol += add (n floats)
+__global__ void __launch_bounds__(256) back_add(float* __restrict__ vol, const float* __restrict__ add, long n)
+{
+    for (long i = blockIdx.x * 256L + threadIdx.x; i < n; i += (long)gridDim.x * 256)
+        vol[i] += add[i];
 }
 
 // ---------------------------------------------------------------------------
@@ -124,12 +139,14 @@ static void launch_proj_walk(const DevGeom& g, const T* cz, float* d_prj, hipStr
 }
 
 template <typename T, int NM>
-static void launch_back_walk(const DevGeom& g, const T* P, float* d_vol, const int* views, int nv, int yslab,
-                             int acc, const ColRec* rec, const unsigned short* start, int SJ, int SR, hipStream_t s)
+static void launch_back_walk(const DevGeom& g, const T* P, float* d_vol, float* d_vol_x, const int* vy, int ny,
+                             const int* vx, int nx, const ColRec* rec, const unsigned short* start, int SJ, int SR,
+                             hipStream_t s)
 {
-    const int NS = yslab ? g.YN : g.XN, NR = yslab ? g.XN : g.YN, ntr = (NR + NM - 1) / NM;
-    hipLaunchKernelGGL((dd_back_walk<T, NM>), dim3((unsigned)(ntr * ((NS + 3) / 4))), dim3(256), 0, s, g, P, d_vol,
-                       views, nv, yslab, acc, NS, NR, ntr, rec, start, SJ, SR);
+    const int ntr_y = (g.XN + NM - 1) / NM, ntr_x = (g.YN + NM - 1) / NM;
+    const int nby = ntr_y * ((g.YN + 3) / 4), nbx = ntr_x * ((g.XN + 3) / 4);
+    hipLaunchKernelGGL((dd_back_walk<T, NM>), dim3((unsigned)(nby + nbx)), dim3(256), 0, s, g, P, d_vol, d_vol_x,
+                       vy, ny, vx, nx, nby, ntr_y, ntr_x, rec, start, SJ, SR);
 }
 
 template <typename T>
@@ -157,7 +174,7 @@ hipError_t launch_back_start(const DevGeom& g, const void* d_rec, unsigned short
 }
 
 template <typename T>
-hipError_t launch_back(const DevGeom& g, const float* d_prj, void* d_P, float* d_vol,
+hipError_t launch_back(const DevGeom& g, const float* d_prj, void* d_P, float* d_vol, float* d_scratch,
                        const int* d_views_y, int ny, const int* d_views_x, int nx, const void* d_rec,
                        const unsigned short* d_start, int SJ, int SR, hipStream_t s)
 {
@@ -166,25 +183,23 @@ hipError_t launch_back(const DevGeom& g, const float* d_prj, void* d_P, float* d
     const long ncu = (long)g.PN * g.DNU;
     hipLaunchKernelGGL(prjprefix_kernel<T>, dim3((unsigned)((ncu + 3) / 4)), dim3(256), 0, s,
                        g, d_prj, P);
-    // pass 1: y-slab views (slab = y, rows along x) overwrite; pass 2: x-slab views accumulate
-    for (int pass = 0; pass < 2; ++pass) {
-        const int ys = pass == 0;
-        const int* vw = ys ? d_views_y : d_views_x;
-        const int nv = ys ? ny : nx;
-        switch (g.tile_nm) {
-        case 8: launch_back_walk<T, 8>(g, P, d_vol, vw, nv, ys, pass, rec, d_start, SJ, SR, s); break;
-        case 12: launch_back_walk<T, 12>(g, P, d_vol, vw, nv, ys, pass, rec, d_start, SJ, SR, s); break;
-        default: launch_back_walk<T, 16>(g, P, d_vol, vw, nv, ys, pass, rec, d_start, SJ, SR, s); break;
-        }
+    // y-slab views (slab = y, rows along x) into d_vol, x-slab views into d_scratch, one launch
+    switch (g.tile_nm) {
+    case 8: launch_back_walk<T, 8>(g, P, d_vol, d_scratch, d_views_y, ny, d_views_x, nx, rec, d_start, SJ, SR, s); break;
+    case 12: launch_back_walk<T, 12>(g, P, d_vol, d_scratch, d_views_y, ny, d_views_x, nx, rec, d_start, SJ, SR, s); break;
+    default: launch_back_walk<T, 16>(g, P, d_vol, d_scratch, d_views_y, ny, d_views_x, nx, rec, d_start, SJ, SR, s); break;
     }
+    const long nvox = (long)g.XN * g.YN * g.ZN;
+    hipLaunchKernelGGL(back_add, dim3((unsigned)std::min<long>((nvox + 255) / 256, 8192)), dim3(256), 0, s, d_vol,
+                       d_scratch, nvox);
     return hipGetLastError();
 }
 
 template hipError_t launch_proj<double>(const DevGeom&, const float*, void*, float*, hipStream_t);
 template hipError_t launch_proj<float>(const DevGeom&, const float*, void*, float*, hipStream_t);
-template hipError_t launch_back<double>(const DevGeom&, const float*, void*, float*, const int*, int,
+template hipError_t launch_back<double>(const DevGeom&, const float*, void*, float*, float*, const int*, int,
                                         const int*, int, const void*, const unsigned short*, int, int, hipStream_t);
-template hipError_t launch_back<float>(const DevGeom&, const float*, void*, float*, const int*, int,
+template hipError_t launch_back<float>(const DevGeom&, const float*, void*, float*, float*, const int*, int,
                                        const int*, int, const void*, const unsigned short*, int, int, hipStream_t);
 
 }  // namespace ctlab

@@ -411,11 +411,13 @@ int plan_back(ctlab_plan* P, const float* d_prj, float* d_vol, hipStream_t s)
     const int ny = (int)P->views_y.size(), nx = (int)P->views_x.size();
     const int* vy = P->d_views;
     const int* vx = P->d_views + ny;
+    // the integral-image workspace (>= XN*YN*ZN floats) is idle during backprojection
+    float* scratch = static_cast<float*>(P->d_cz);
     hipError_t e = P->sat_bits == 32
-                       ? launch_back<float>(g, d_prj, P->d_P, d_vol, vy, ny, vx, nx, P->d_rec, P->d_start, P->SJ,
-                                            P->SR, s)
-                       : launch_back<double>(g, d_prj, P->d_P, d_vol, vy, ny, vx, nx, P->d_rec, P->d_start, P->SJ,
-                                             P->SR, s);
+                       ? launch_back<float>(g, d_prj, P->d_P, d_vol, scratch, vy, ny, vx, nx, P->d_rec, P->d_start,
+                                            P->SJ, P->SR, s)
+                       : launch_back<double>(g, d_prj, P->d_P, d_vol, scratch, vy, ny, vx, nx, P->d_rec,
+                                             P->d_start, P->SJ, P->SR, s);
     if (hip_check(e, "backprojector launch")) return t_err;
     return CTLAB_OK;
 }

@@ -22,8 +22,9 @@ struct GeomIn {
 
 template <typename T>
 hipError_t launch_proj(const DevGeom& g, const float* d_vol, void* d_cz, float* d_prj, hipStream_t s);
+// d_scratch: XN*YN*ZN floats of work space (the x-slab part before it is added)
 template <typename T>
-hipError_t launch_back(const DevGeom& g, const float* d_prj, void* d_P, float* d_vol,
+hipError_t launch_back(const DevGeom& g, const float* d_prj, void* d_P, float* d_vol, float* d_scratch,
                        const int* d_views_y, int ny, const int* d_views_x, int nx, const void* d_rec,
                        const unsigned short* d_start, int SJ, int SR, hipStream_t s);
 hipError_t launch_back_start(const DevGeom& g, const void* d_rec, unsigned short* d_start, int SJ, int SR,

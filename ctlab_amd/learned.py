@@ -259,7 +259,7 @@ def fista_tv_run(rc, L, b_shape_vol, iters, tv_lambda, tv_iters, truth, zsl, dev
     in_loop = learned is not None and feedback
     out = x                               # the output of the current iteration
     f_out = f_x
-    cand_psnr = None
+    cand_psnr = cand_fid = None
     hist = []
     taken = {}
     elapsed = 0.0
@@ -269,9 +269,9 @@ def fista_tv_run(rc, L, b_shape_vol, iters, tv_lambda, tv_iters, truth, zsl, dev
             rc.gradient(y, g)
             z.copy_(y)
             RC.axpby(-1.0 / L, g, 1.0, z)
-            nonlocal out, f_out, cand_psnr
+            nonlocal out, f_out, cand_psnr, cand_fid
             accepted = False
-            cand_psnr = None
+            cand_psnr = cand_fid = None
             tried = learned is not None and ((k + 1) % learned_every == 0 or k + 1 == iters)
             if not in_loop or not tried:
                 RC.tv_prox_into(z, x, tv_lambda / L, tv_iters, shape, work)
@@ -283,8 +283,10 @@ def fista_tv_run(rc, L, b_shape_vol, iters, tv_lambda, tv_iters, truth, zsl, dev
                     if f_c <= f_x or (noise_level is not None and f_c <= noise_level):
                         out, f_out = xc, f_c
                         accepted = True
-                    elif truth is not None:
-                        cand_psnr = psnr(xc, truth, zsl)
+                    else:
+                        cand_fid = f_c
+                        if truth is not None:
+                            cand_psnr = psnr(xc, truth, zsl)
             else:
                 RC.tv_prox_into(z, xt, tv_lambda / L, tv_iters, shape, work)
                 learned(xt, xc)
@@ -294,6 +296,7 @@ def fista_tv_run(rc, L, b_shape_vol, iters, tv_lambda, tv_iters, truth, zsl, dev
                     f_x = f_c
                     accepted = True
                 else:
+                    cand_fid = f_c
                     if truth is not None:
                         cand_psnr = psnr(xc, truth, zsl)
                     x.copy_(xt)
@@ -313,7 +316,8 @@ def fista_tv_run(rc, L, b_shape_vol, iters, tv_lambda, tv_iters, truth, zsl, dev
         if snaps is not None and (k + 1) in snaps:
             taken[k + 1] = x.clone()
         hist.append({"iter": k + 1, "t": elapsed, "psnr": psnr(out, truth, zsl) if truth is not None else None,
-                     "fidelity": f_out, "learned": acc, "rejected_candidate_psnr": cand_psnr})
+                     "fidelity": f_out, "learned": acc, "rejected_candidate_psnr": cand_psnr,
+                     "rejected_candidate_fidelity": cand_fid})
     return out, hist, taken
 
 
