@@ -365,6 +365,21 @@ def cfg3_bench(args):
     print(json.dumps(out))
 
 
+def cnn_bench(args):
+    """CNN kernels alone: forward at 512x512x64 and a training step at 256x256x32."""
+    import torch
+    from ctlab_amd import cnn as CN
+    from ctlab_amd import learned as LE
+    dev = torch.device("cuda:0")
+    params = CN.init_params(0)
+    rf, tf = LE.cnn_forward_rate((512, 512, 64), params, dev, reps=args.steps)
+    rt, tt = LE.cnn_train_step_rate((256, 256, 32), params, dev, reps=args.steps)
+    peak = 2.5e15
+    print(json.dumps({"metric": METRIC, "mode": "learned prox CNN kernels (BF16 MFMA, FP32 accumulate)",
+                      "forward_512x512x64": {"s": tf, "tflops": rf / 1e12, "frac_of_bf16_dense_peak": rf / peak},
+                      "train_step_256x256x32": {"s": tt, "tflops": rt / 1e12, "frac_of_bf16_dense_peak": rt / peak}}))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -378,6 +393,7 @@ def main():
     ap.add_argument("--learned", action="store_true", help="cfg5: learned FISTA vs plain FISTA-TV")
     ap.add_argument("--batch", action="store_true", help="cfg4: batched OS-SART over uneven problems")
     ap.add_argument("--cfg3", action="store_true", help="cfg3: OS-SART and FISTA-TV on the 2-D fan geometry")
+    ap.add_argument("--cnn", action="store_true", help="learned-prox CNN kernels alone (forward, training step)")
     ap.add_argument("--problems", type=int, default=32)
     ap.add_argument("--subsets", type=int, default=10)
     ap.add_argument("--samples", type=int, default=2000)
@@ -398,6 +414,8 @@ def main():
         return batch_bench(args)
     if args.cfg3:
         return cfg3_bench(args)
+    if args.cnn:
+        return cnn_bench(args)
 
     import torch
     from ctlab_amd import dd

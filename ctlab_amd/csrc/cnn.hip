@@ -19,8 +19,8 @@ __global__ void k_dgrad_last(const float*, const float*, const __bf16*, __bf16*,
 __global__ void k_wgrad32(const __bf16*, const __bf16*, float*, int, int, int, long);
 __global__ void k_wgrad_first(const __bf16*, const float*, float*, int, int, int, long);
 __global__ void k_wgrad_last(const float*, const __bf16*, float*, int, int, int, long);
-__global__ void k_bias_grad_cm(const __bf16*, long, float*);
-__global__ void k_sum_f32(const float*, long, float*);
+__global__ void k_bias_grad_partial(const __bf16*, long, long, float*);
+__global__ void k_sum_partial(const float*, long, long, float*);
 __global__ void k_reduce_chunks(const float*, int, int, float*);
 __global__ void k_flip_transpose(const __bf16*, __bf16*);
 __global__ void k_adam(float*, const float*, float*, float*, long, float, float, float, float, float, float);
@@ -127,7 +127,9 @@ int backward(ctlab_cnn* N, const float* x, hipStream_t s)
                        ZN, kVecChunk);
     hipLaunchKernelGGL(k_reduce_chunks, dim3(blocks(27 * CH, 256)), dim3(256), 0, s, N->part, (int)N->nvchunks,
                        27 * CH, G + kOffW[4]);
-    hipLaunchKernelGGL(k_sum_f32, dim3(1), dim3(256), 0, s, N->g5, n, G + kOffW[4] + kLenW[4]);
+    hipLaunchKernelGGL(k_sum_partial, dim3((unsigned)N->nvchunks), dim3(256), 0, s, N->g5, n, kVecChunk, N->part);
+    hipLaunchKernelGGL(k_reduce_chunks, dim3(1), dim3(256), 0, s, N->part, (int)N->nvchunks, 1,
+                       G + kOffW[4] + kLenW[4]);
     hipLaunchKernelGGL(k_dgrad_last, dim3(blocks(n, 256)), dim3(256), 0, s, N->g5, N->wr + 27 * CH, N->a_cl[3],
                        N->g_cl[0], N->g_cm[0], XN, YN, ZN);
     int cur = 0;   // g_*[cur] holds dL/dz of layer l+1 (l = 3, 2, 1 below: layers 4, 3, 2)
@@ -137,18 +139,24 @@ int backward(ctlab_cnn* N, const float* x, hipStream_t s)
                            N->part, XN, YN, ZN, kWgradChunk);
         hipLaunchKernelGGL(k_reduce_chunks, dim3(blocks(kW32, 256)), dim3(256), 0, s, N->part, (int)N->nchunks,
                            (int)kW32, G + kOffW[l]);
-        hipLaunchKernelGGL(k_bias_grad_cm, dim3(CH), dim3(256), 0, s, N->g_cm[cur], n, G + kOffW[l] + kLenW[l]);
+        hipLaunchKernelGGL(k_bias_grad_partial, dim3((unsigned)N->nvchunks), dim3(256), 0, s, N->g_cm[cur], n,
+                           kVecChunk, N->part);
+        hipLaunchKernelGGL(k_reduce_chunks, dim3(1), dim3(256), 0, s, N->part, (int)N->nvchunks, CH,
+                           G + kOffW[l] + kLenW[l]);
         // input gradient through layer l+1, masked by the ReLU of layer l
         hipLaunchKernelGGL(k_conv32, dim3(blocks(n, 256)), dim3(256), 0, s, N->g_cl[cur], N->wt + (l - 1) * kW32,
                            (const float*)nullptr, N->a_cl[l - 1], N->g_cl[cur ^ 1], N->g_cm[cur ^ 1], XN, YN, ZN, 0);
         cur ^= 1;
     }
     // layer 1
-    hipLaunchKernelGGL(k_wgrad_first, dim3((unsigned)N->nvchunks), dim3(256), 0, s, N->g_cm[cur], x, N->part, XN, YN,
+    hipLaunchKernelGGL(k_wgrad_first, dim3((unsigned)N->nvchunks), dim3(256), 0, s, N->g_cl[cur], x, N->part, XN, YN,
                        ZN, kVecChunk);
     hipLaunchKernelGGL(k_reduce_chunks, dim3(blocks(27 * CH, 256)), dim3(256), 0, s, N->part, (int)N->nvchunks,
                        27 * CH, G + kOffW[0]);
-    hipLaunchKernelGGL(k_bias_grad_cm, dim3(CH), dim3(256), 0, s, N->g_cm[cur], n, G + kOffW[0] + kLenW[0]);
+    hipLaunchKernelGGL(k_bias_grad_partial, dim3((unsigned)N->nvchunks), dim3(256), 0, s, N->g_cm[cur], n, kVecChunk,
+                       N->part);
+    hipLaunchKernelGGL(k_reduce_chunks, dim3(1), dim3(256), 0, s, N->part, (int)N->nvchunks, CH,
+                       G + kOffW[0] + kLenW[0]);
     return chk(hipGetLastError(), "cnn backward");
 }
 }  // namespace

@@ -349,9 +349,11 @@ k_wgrad32(const __bf16* __restrict__ g_cm, const __bf16* __restrict__ a_cm, floa
 }
 
 // weight gradient of the first layer: dW1[t][co] = sum_v g1[v][co] * x[v + off(t)]
-// one block per chunk; thread (t, co) pairs; partials part[chunk][27*32]
+// one block per chunk; thread (t, co) pairs; partials part[chunk][27*32].  The
+// gradient is read channel-last (g_cl[v][co]): the 64 lanes of a wavefront,
+// two taps x 32 channels, then read one 64-byte line per voxel.
 __global__ void __launch_bounds__(256)
-k_wgrad_first(const __bf16* __restrict__ g_cm, const float* __restrict__ x, float* __restrict__ part,
+k_wgrad_first(const __bf16* __restrict__ g_cl, const float* __restrict__ x, float* __restrict__ part,
               int XN, int YN, int ZN, long chunk)
 {
     const long nvox = (long)XN * YN * ZN;
@@ -365,7 +367,7 @@ k_wgrad_first(const __bf16* __restrict__ g_cm, const float* __restrict__ x, floa
             const int idx = threadIdx.x + 256 * k;   // t*32 + co
             if (idx >= 27 * CH) break;
             const int t = idx / CH, co = idx % CH;
-            if (tap_ok(t, y, xx, z, YN, XN, ZN)) acc[k] += bf(g_cm[(long)co * nvox + v]) * x[v + tap_off(t, XN, ZN)];
+            if (tap_ok(t, y, xx, z, YN, XN, ZN)) acc[k] += bf(g_cl[v * CH + co]) * x[v + tap_off(t, XN, ZN)];
         }
     }
     for (int k = 0; k < 4; ++k) {
@@ -400,29 +402,35 @@ k_wgrad_last(const float* __restrict__ g5, const __bf16* __restrict__ a_cl, floa
     }
 }
 
-// bias gradients: db[c] = sum_v g_cm[c][v]  (one block per channel)
+// bias-gradient partials: part[j][c] = sum over chunk j of g_cm[c][v] (block j;
+// wavefront w takes channels w, w+4, ...; lanes read consecutive voxels).  The
+// chunks are then summed in a fixed order (k_reduce_chunks), so the result
+// does not depend on scheduling.
 __global__ void __launch_bounds__(256)
-k_bias_grad_cm(const __bf16* __restrict__ g_cm, long nvox, float* __restrict__ db)
+k_bias_grad_partial(const __bf16* __restrict__ g_cm, long nvox, long chunk, float* __restrict__ part)
 {
-    __shared__ float sh[4];
-    float s = 0.0f;
-    for (long v = threadIdx.x; v < nvox; v += 256) s += bf(g_cm[(long)blockIdx.x * nvox + v]);
-    for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
-    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = s;
-    __syncthreads();
-    if (threadIdx.x == 0) db[blockIdx.x] = sh[0] + sh[1] + sh[2] + sh[3];
+    const long c0 = blockIdx.x * chunk, c1 = min(c0 + chunk, nvox);
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    for (int c = wv; c < CH; c += 4) {
+        float s = 0.0f;
+        for (long v = c0 + lane; v < c1; v += 64) s += bf(g_cm[(long)c * nvox + v]);
+        for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
+        if (lane == 0) part[(long)blockIdx.x * CH + c] = s;
+    }
 }
 
+// sum partials: part[j] = sum over chunk j of x[v]
 __global__ void __launch_bounds__(256)
-k_sum_f32(const float* __restrict__ x, long n, float* __restrict__ out)
+k_sum_partial(const float* __restrict__ x, long n, long chunk, float* __restrict__ part)
 {
     __shared__ float sh[4];
+    const long c0 = blockIdx.x * chunk, c1 = min(c0 + chunk, n);
     float s = 0.0f;
-    for (long v = threadIdx.x; v < n; v += 256) s += x[v];
+    for (long v = c0 + threadIdx.x; v < c1; v += 256) s += x[v];
     for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
     if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = s;
     __syncthreads();
-    if (threadIdx.x == 0) out[0] = sh[0] + sh[1] + sh[2] + sh[3];
+    if (threadIdx.x == 0) part[blockIdx.x] = sh[0] + sh[1] + sh[2] + sh[3];
 }
 
 // out[i] = sum_c part[c][i]  (fixed order over chunks)

@@ -339,6 +339,22 @@ def time_to_psnr(hist, target):
     return None, None, None
 
 
+def cnn_train_step_rate(shape, params, dev, reps=3):
+    """FLOP/s of one training step (forward + input and weight gradients + Adam;
+    3 x 169,344 FLOP per voxel for the convolutions)."""
+    import torch
+    net = CN.CNN(shape, training=True, params=params)
+    x = torch.rand(shape, device=dev)
+    t = torch.rand(shape, device=dev)
+    net.loss_grad(x, t)
+    net.adam_step(1e-4)
+    best = 1e30
+    for _ in range(reps):
+        _, dt = _timed(lambda: (net.loss_grad(x, t), net.adam_step(1e-4)))
+        best = min(best, dt)
+    return 3 * 169344.0 * x.numel() / best, best
+
+
 def cnn_forward_rate(shape, params, dev, reps=3):
     """FLOP/s of one network forward at this volume shape (169,344 FLOP per voxel)."""
     import torch
