@@ -45,6 +45,9 @@ __device__ __forceinline__ float bf(const __bf16 v) { return (float)v; }
 //   out = act(conv(in; w) + bias) ; act: relu (relu=1) or identity
 //   mask (optional, channels-last): out *= (mask > 0)   [input-gradient pass]
 // Writes channels-last and (optionally) channel-major copies.
+// The weights pass through LDS one dy plane (9 taps, 18 KB) at a time, so
+// that several blocks fit on a CU, and each tap's activation loads are issued
+// while the previous tap's MFMAs run.
 // ---------------------------------------------------------------------------
 constexpr int kConvMB = 4;           // 16-voxel blocks per wavefront
 
@@ -53,10 +56,8 @@ k_conv32(const __bf16* __restrict__ in, const __bf16* __restrict__ w, const floa
          const __bf16* __restrict__ mask, __bf16* __restrict__ out_cl, __bf16* __restrict__ out_cm,
          int XN, int YN, int ZN, int relu)
 {
-    __shared__ bf16x8 sW[27 * CH * CH / 8];        // 55,296 B
+    __shared__ bf16x8 sW[9 * CH * CH / 8];         // 18,432 B: the 9 taps of one dy plane
     const long nvox = (long)XN * YN * ZN;
-    for (int i = threadIdx.x; i < 27 * CH * CH / 8; i += 256) sW[i] = ((const bf16x8*)w)[i];
-    __syncthreads();
 
     const int lane = threadIdx.x & 63;
     const int wv = threadIdx.x >> 6;
@@ -79,17 +80,34 @@ k_conv32(const __bf16* __restrict__ in, const __bf16* __restrict__ w, const floa
 #pragma unroll
     for (int b = 0; b < kConvMB; ++b) acc[b][0] = acc[b][1] = (f32x4){0.f, 0.f, 0.f, 0.f};
 
-    for (int t = 0; t < 27; ++t) {
+    auto load_a = [&](int t, bf16x8 (&a)[kConvMB]) {
         const int off = tap_off(t, XN, ZN);
-        const bf16x8 b0 = sW[((t * CH) + m) * (CH / 8) + kq];
-        const bf16x8 b1 = sW[((t * CH) + 16 + m) * (CH / 8) + kq];
 #pragma unroll
         for (int b = 0; b < kConvMB; ++b) {
-            bf16x8 a = (bf16x8){0, 0, 0, 0, 0, 0, 0, 0};
+            a[b] = (bf16x8){0, 0, 0, 0, 0, 0, 0, 0};
             if (vv[b] < nvox && tap_ok(t, vy[b], vx[b], vz[b], YN, XN, ZN))
-                a = *(const bf16x8*)(in + (vv[b] + off) * CH + 8 * kq);
-            acc[b][0] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b0, acc[b][0], 0, 0, 0);
-            acc[b][1] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b1, acc[b][1], 0, 0, 0);
+                a[b] = *(const bf16x8*)(in + (vv[b] + off) * CH + 8 * kq);
+        }
+    };
+    bf16x8 a_cur[kConvMB], a_nxt[kConvMB];
+    load_a(0, a_cur);
+    for (int plane = 0; plane < 3; ++plane) {
+        __syncthreads();   // the previous plane's weights are no longer read
+        for (int i = threadIdx.x; i < 9 * CH * CH / 8; i += 256)
+            sW[i] = ((const bf16x8*)w)[plane * 9 * CH * CH / 8 + i];
+        __syncthreads();
+        for (int tt = 0; tt < 9; ++tt) {
+            const int t = plane * 9 + tt;
+            if (t + 1 < 27) load_a(t + 1, a_nxt);
+            const bf16x8 b0 = sW[((tt * CH) + m) * (CH / 8) + kq];
+            const bf16x8 b1 = sW[((tt * CH) + 16 + m) * (CH / 8) + kq];
+#pragma unroll
+            for (int b = 0; b < kConvMB; ++b) {
+                acc[b][0] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a_cur[b], b0, acc[b][0], 0, 0, 0);
+                acc[b][1] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a_cur[b], b1, acc[b][1], 0, 0, 0);
+            }
+#pragma unroll
+            for (int b = 0; b < kConvMB; ++b) a_cur[b] = a_nxt[b];
         }
     }
 
