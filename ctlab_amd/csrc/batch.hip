@@ -160,7 +160,7 @@ __global__ void __launch_bounds__(256) bk_back(const BatchProb* __restrict__ pb,
 {
     constexpr int yslab = YSLAB;
     __shared__ T sG[4][72];
-    __shared__ T sA[4][NM * 64];
+    __shared__ typename BackAcc<T, NM>::type sA[4][NM * 64];
     const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int lane = threadIdx.x & 63;
     drain<1>(q, [&](int k, int it) {

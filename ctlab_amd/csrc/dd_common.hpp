@@ -60,6 +60,8 @@ struct DevGeom {
     const CTLAB_GMEM double* ez;         // DNV: row centre z - z0 (0 for fan beam)
     const CTLAB_GMEM double* rowfac;     // DNV: 1/(zhat[v+1]-zhat[v]) (cone) or 1 (fan)
     const CTLAB_GMEM unsigned char* mask;// XN*YN or nullptr
+    const CTLAB_GMEM double* invs;  // PN*sjs: 1 / (slab centre - source slab coordinate), slab j of view p
+    int sjs;                  // row stride of invs (max(XN, YN))
     double zhat0, dzh;        // uniform row model used by the backprojector
     int tile_nu;              // projector: detector columns per wavefront (8, 12, 16)
     int tile_nm;              // backprojector: voxel rows per wavefront (8, 12, 16)

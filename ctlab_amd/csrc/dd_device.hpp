@@ -123,6 +123,7 @@ __device__ __forceinline__ void proj_walk_wave(const DevGeom& g, const CTLAB_GME
     for (int q = 0; q < NU; ++q) acc[q] = T(0);
     T acc_top = T(0);   // lane s: row 63's top-boundary term of column step s
 
+    const CTLAB_GMEM double* __restrict__ invs = g.invs + (long)p * g.sjs;
     for (int j = 0; j < vp.NS; ++j) {
         const double sj = vp.s0 + j * g.dx;
         const double rs = sj * inv_dx;
@@ -134,9 +135,10 @@ __device__ __forceinline__ void proj_walk_wave(const DevGeom& g, const CTLAB_GME
         const int m_l = min((int)a_l, NR - 1);
         const double f_l = a_l - (double)m_l;
         // columns (lane = tile column u, boundaries u and u+1) with a positive row extent
-        const double a_n = __shfl_down(a_l, 1, 64);
+        const double a_n = shift_down1(a_l);
         const unsigned long long act = __ballot(lane < nu && (vp.asc ? a_n > a_l : a_l > a_n));
-        const T qj = (T)(g.cone ? 1.0 / (sj * sj) : 1.0 / fabs(sj));
+        const double inv_sj = invs[j];
+        const T qj = (T)(g.cone ? inv_sj * inv_sj : fabs(inv_sj));
         const double t_l = g.cone ? sj * tau_l : 1.0;     // z magnification of column `lane`
         const CTLAB_GMEM T* __restrict__ base = cz + j * slab_off + zl;
         int m = __builtin_amdgcn_readlane(m_l, b0);
@@ -280,19 +282,26 @@ struct ColRec { double klo, khi, invdk, esdz; };
 // Backprojector walk of one (slab j, tile rt of NM rows) over the views of one
 // slab orientation; lane = z.  acc[i] (zeroed here) receives voxel row
 // rt*NM + i.  See dd_back_walk for the structure.
+// Row accumulator type in LDS: FP64 up to 12-row tiles; FP32 for 16-row tiles,
+// whose FP64 accumulators (8 KB per wavefront) would cap occupancy at 4
+// wavefronts per SIMD.  Each view's contribution is still formed in FP64.
+template <typename T, int NM> struct BackAcc { typedef T type; };
+template <> struct BackAcc<double, 16> { typedef float type; };
+
 template <typename T, int NM>
 __device__ __forceinline__ void back_walk_accum(const DevGeom& g, const CTLAB_GMEM T* __restrict__ P,
                                                 const CTLAB_GMEM int* __restrict__ views, int nviews, int NR, int j,
                                                 int rt, const CTLAB_GMEM ColRec* __restrict__ rec,
                                                 const CTLAB_GMEM unsigned short* __restrict__ start,
-                                                int SJ, int SR, int lane, T* G, T* A, T (&acc)[NM])
+                                                int SJ, int SR, int lane, T* G,
+                                                typename BackAcc<T, NM>::type* A, T (&acc)[NM])
 {
     if (lane == 0) G[0] = T(0);
     // row accumulators live in LDS (A[i*64 + lane]): the row index varies at run
     // time, and indexing registers with it costs four scalar-set / move pairs
     // per FP64 update
 #pragma unroll
-    for (int i = 0; i < NM; ++i) A[i * 64 + lane] = T(0);
+    for (int i = 0; i < NM; ++i) A[i * 64 + lane] = 0;
     const int r0 = rt * NM;
     const int nm = min(NM, NR - r0);
     const int ZN = g.ZN, DNU = g.DNU, DNV = g.DNV;
@@ -315,7 +324,7 @@ __device__ __forceinline__ void back_walk_accum(const DevGeom& g, const CTLAB_GM
         const ViewParams vp = g.vp[p];
         const double sj = vp.s0 + j * g.dx;
         const double rs = sj * inv_dx;
-        const double inv_sj = 1.0 / sj;
+        const double inv_sj = g.invs[(long)p * g.sjs + j];
         const double inv_rs = fabs(g.dx * inv_sj);
         const double lzb = ((double)lane - vp.beta0) * inv_sj;
         // walk-order column c: its record rp[0] and v-prefix row pp[0]; the next
@@ -356,12 +365,12 @@ __device__ __forceinline__ void back_walk_accum(const DevGeom& g, const CTLAB_GM
             }
             const double fr = fmin(fmax((mp - alo) * iw, 0.0), 1.0);
             const T W = Tsum + (T)fr * D;
-            if (i > 0) A[(i - 1) * 64 + lane] += W - Wprev;
+            if (i > 0) A[(i - 1) * 64 + lane] += (typename BackAcc<T, NM>::type)(W - Wprev);
             Wprev = W;
         }
     }
 #pragma unroll
-    for (int i = 0; i < NM; ++i) acc[i] = A[i * 64 + lane];
+    for (int i = 0; i < NM; ++i) acc[i] = (T)A[i * 64 + lane];
 }
 
 // Writes one backprojected tile (lane = z): for voxel row i of the tile,
@@ -392,7 +401,7 @@ __device__ __forceinline__ void back_walk_wave(const DevGeom& g, const CTLAB_GME
                                                int nviews, int yslab, int accumulate, int NR, int j, int rt,
                                                const CTLAB_GMEM ColRec* __restrict__ rec,
                                                const CTLAB_GMEM unsigned short* __restrict__ start, int SJ, int SR, int lane,
-                                               T* G, T* A)
+                                               T* G, typename BackAcc<T, NM>::type* A)
 {
     T acc[NM];
     back_walk_accum<T, NM>(g, P, views, nviews, NR, j, rt, rec, start, SJ, SR, lane, G, A, acc);

@@ -109,7 +109,7 @@ dd_back_walk(DevGeom g, const T* __restrict__ P, float* __restrict__ vol_y, floa
              int SJ, int SR)
 {
     __shared__ T sG[4][72];
-    __shared__ T sA[4][NM * 64];
+    __shared__ typename BackAcc<T, NM>::type sA[4][NM * 64];
     const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int ys = blockIdx.x < (unsigned)nby;
     const int b = ys ? blockIdx.x : blockIdx.x - nby;

@@ -149,6 +149,7 @@ static int build_tables(ctlab_plan* P)
     P->exy2.resize((size_t)PN * DNU);
     P->invdk.resize((size_t)PN * DNU);
     P->esdz.resize((size_t)PN * DNU);
+    P->invs.assign((size_t)PN * std::max(g.XN, g.YN), 0.0);
     P->rec.resize((size_t)PN * DNU * 4);
     if (DNU >= 0xFFFF) {
         set_error(CTLAB_ERR_GEOMETRY, "DNU (%d) must be below 65535", DNU);
@@ -215,6 +216,8 @@ static int build_tables(ctlab_plan* P)
                 g.cone ? g.dx * g.dx / std::fabs(dk) : g.dx * g.dx * std::fabs(t) / std::fabs(dk);
         }
         V.asc = (V.s0 * dk0) > 0;
+        const int sjs = std::max(g.XN, g.YN);
+        for (int j = 0; j < V.NS; ++j) P->invs[(size_t)p * sjs + j] = 1.0 / (V.s0 + j * g.dx);
         // row coordinate of the top voxel boundary, (ZN - beta0) * esdz / sj - zhat0 / dzh, is
         // bilinear in (esdz, 1/sj): its corners bound it over all slabs and columns
         V.top_total = 0;
@@ -260,7 +263,7 @@ void free_plan(ctlab_plan* P)
     hipGetDevice(&cur);
     hipSetDevice(P->device);
     if (P->ws_shared) P->d_cz = P->d_P = nullptr;
-    void* ptrs[] = {P->d_vp, P->d_kap, P->d_tau, P->d_colfac, P->d_exy2, P->d_invdk, P->d_esdz, P->d_zhat, P->d_ez, P->d_rec, P->d_start,
+    void* ptrs[] = {P->d_vp, P->d_kap, P->d_tau, P->d_colfac, P->d_exy2, P->d_invdk, P->d_esdz, P->d_invs, P->d_zhat, P->d_ez, P->d_rec, P->d_start,
                     P->d_rowfac, P->d_mask, P->d_views, P->d_cz, P->d_P};
     for (void* q : ptrs)
         if (q) hipFree(q);
@@ -308,7 +311,7 @@ ctlab_plan* finish_plan(ctlab_plan* P, int gpunum, const ctlab_plan* share_ws)
     views.insert(views.end(), P->views_x.begin(), P->views_x.end());
     if (upload(&P->d_vp, P->vp) || upload(&P->d_kap, P->kap) || upload(&P->d_tau, P->tau) ||
         upload(&P->d_colfac, P->colfac) || upload(&P->d_exy2, P->exy2) || upload(&P->d_invdk, P->invdk) ||
-        upload(&P->d_esdz, P->esdz) || upload(&P->d_zhat, P->zhat) ||
+        upload(&P->d_esdz, P->esdz) || upload(&P->d_zhat, P->zhat) || upload(&P->d_invs, P->invs) ||
         upload(&P->d_ez, P->ez) || upload(&P->d_rowfac, P->rowfac) || upload(&P->d_mask, P->in.mask) ||
         upload(&P->d_views, views) || upload(&P->d_rec, P->rec) ||
         (can_share(P, share_ws) ? (P->d_cz = share_ws->d_cz, P->d_P = share_ws->d_P, P->ws_elem = share_ws->ws_elem,

@@ -36,13 +36,13 @@ struct ctlab_plan {
     int device = 0;
     ctlab::GeomIn in;
     std::vector<ctlab::ViewParams> vp;
-    std::vector<double> kap, tau, colfac, exy2, invdk, esdz, zhat, ez, rowfac;
+    std::vector<double> kap, tau, colfac, exy2, invdk, esdz, zhat, ez, rowfac, invs;
     double zhat0 = 0, dzh = 1;
     std::vector<int> views_y, views_x;
     // device copies
     ctlab::ViewParams* d_vp = nullptr;
     double *d_kap = nullptr, *d_tau = nullptr, *d_colfac = nullptr, *d_exy2 = nullptr;
-    double *d_invdk = nullptr, *d_esdz = nullptr;
+    double *d_invdk = nullptr, *d_esdz = nullptr, *d_invs = nullptr;
     double *d_zhat = nullptr, *d_ez = nullptr, *d_rowfac = nullptr;
     unsigned char* d_mask = nullptr;
     int* d_views = nullptr;
@@ -67,6 +67,8 @@ struct ctlab_plan {
         g.invdk = ctlab::gmem(d_invdk); g.esdz = ctlab::gmem(d_esdz);
         g.zhat = ctlab::gmem(d_zhat); g.ez = ctlab::gmem(d_ez); g.rowfac = ctlab::gmem(d_rowfac);
         g.mask = in.mask.empty() ? nullptr : ctlab::gmem(d_mask);
+        g.invs = ctlab::gmem(d_invs);
+        g.sjs = in.XN > in.YN ? in.XN : in.YN;
         g.zhat0 = zhat0; g.dzh = dzh;
         g.tile_nu = tile_nu; g.tile_nm = tile_nm;
         return g;
