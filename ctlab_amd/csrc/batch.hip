@@ -132,11 +132,13 @@ template <typename T, int NU>
 __global__ void __launch_bounds__(256) bk_proj(const BatchProb* __restrict__ pb, Queue q)
 {
     __shared__ T sG[4][NU * kGStride];
+    __shared__ __align__(16) double sFT[4][2 * (NU + 1)];
     const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int lane = threadIdx.x & 63;
     drain<1>(q, [&](int k, int it) {
         const BatchProb b = load_prob(pb + k);
-        proj_walk_wave<T, NU>(b.g, (const CTLAB_GMEM T*)b.cz, b.prj, it % b.g.PN, it / b.g.PN, lane, sG[wv]);
+        proj_walk_wave<T, NU>(b.g, (const CTLAB_GMEM T*)b.cz, b.prj, it % b.g.PN, it / b.g.PN, lane, sG[wv],
+                              sFT[wv]);
     });
 }
 

@@ -41,13 +41,21 @@ __device__ __forceinline__ float shift_down1(float v)
     return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x130, 0xf, 0xf, true));
 }
 
+// clamp i to [0, hi] in one VALU instruction (hi uniform)
+__device__ __forceinline__ int clamp_idx(int i, int hi)
+{
+    int r;
+    asm("v_med3_i32 %0, %1, 0, %2" : "=v"(r) : "v"(i), "s"(hi));
+    return r;
+}
+
 // piecewise-linear read of a prefix table G[0..n] (G[0] = 0) at b, clamped
 // to [0, n]: the cell index is clamped as an integer and the fraction to
 // [0, 1], which gives G[0] below the table and G[n] above it
 template <typename T>
 __device__ __forceinline__ T lds_interp(const T* G, double b, int n)
 {
-    const int i = min(max((int)b, 0), n - 1);
+    const int i = clamp_idx((int)b, n - 1);
     const T fr = (T)fmin(fmax(b - (double)i, 0.0), 1.0);
     const T g0 = G[i];
     const T g1 = G[i + 1];
@@ -86,19 +94,33 @@ constexpr int kGStride = 68;
 
 // Projector walk of one (view p, tile of NU detector columns) over all slabs.
 // Lane = z for the SAT rows, lane = detector row boundary for the outputs;
-// G: NU * kGStride entries of LDS.
+// G: NU * kGStride entries of LDS, FT: 2 * (NU + 1) doubles of LDS (16-byte
+// aligned).
 //
 // Row v needs the footprint z-prefix at its two boundaries v and v+1: each lane
-// interpolates boundary `lane` and takes the upper one from lane+1.  With 64
-// rows the top boundary (64) has no lane: its table stays in LDS (one region
-// per column step) and after the slab's column loop one pass with lane = step
+// interpolates boundary `lane`.  In FP64 the interpolated prefixes are summed
+// over slabs per boundary and differenced once at the end (the sum is linear,
+// and FP64 keeps the cancellation far below the FP32 output); the FP32 variant
+// differences every slab's term before summing.  With 64 rows the top
+// boundary (64) has no lane: its table stays in LDS (one region per column
+// step) and after the slab's column loop one pass with lane = step
 // interpolates all of them; row 63 collects that term lane-parallel in
 // acc_top and adds it in the epilogue.
+//
+// Per slab the boundary fractions and column magnifications go to the step
+// table FT as pairs in walk order (step k: fraction of the boundary it ends
+// at, magnification of the column it crosses), so that every step reads its
+// two uniform operands with one broadcast LDS read.
+//
+// Rows are read two ahead through a pointer that steps one row at a time; the
+// read-ahead runs up to two rows past the slab's last row, which the
+// integral-image buffer provides as padding (2 * (XN + 1) * ZN elements).
 template <typename T, int NU>
 __device__ __forceinline__ void proj_walk_wave(const DevGeom& g, const CTLAB_GMEM T* __restrict__ cz,
                                                CTLAB_GMEM float* __restrict__ prj,
-                                               int p, int tile, int lane, T* G)
+                                               int p, int tile, int lane, T* G, double* FT)
 {
+    constexpr bool kSumThenDiff = sizeof(T) == 8;
     if (lane < NU) G[lane * kGStride] = T(0);
     const ViewParams vp = g.vp[p];
     const int u0 = tile * NU;
@@ -110,14 +132,19 @@ __device__ __forceinline__ void proj_walk_wave(const DevGeom& g, const CTLAB_GME
     const double tau_l = g.tau[(long)p * g.DNU + u0 + min(lane, nu - 1)];
     const int uw = min(lane, nu - 1);
     const double tau_w = g.tau[(long)p * g.DNU + u0 + (vp.asc ? uw : nu - 1 - uw)];
+    // step-table slots: column `lane` is crossed at step kt, boundary `lane`
+    // ends step kf; lanes without a step write the spare slot NU
+    const int kt = lane < nu ? (vp.asc ? lane : nu - 1 - lane) : NU;
+    const int kf0 = vp.asc ? lane - 1 : nu - 1 - lane;
+    const int kf = (lane <= nu && kf0 >= 0) ? kf0 : NU;
     const bool defer_top = DNV == 64;
-    const int zl = min(lane, ZN - 1);
+    // byte offset of lane z within a row (rows are read as uniform row pointer + 32-bit lane offset)
+    const unsigned zoff = (unsigned)min(lane, ZN - 1) * (unsigned)sizeof(T);
     const double zb = g.zhat[min(lane, DNV)];        // boundary `lane`
     const double ztop = g.zhat[DNV];
     const double inv_dx = 1.0 / g.dx;
     const long slab_off = (long)vp.slab_stride * ZN;
     const long row_off = (long)vp.row_stride * ZN;
-
     T acc[NU];
 #pragma unroll
     for (int q = 0; q < NU; ++q) acc[q] = T(0);
@@ -134,59 +161,67 @@ __device__ __forceinline__ void proj_walk_wave(const DevGeom& g, const CTLAB_GME
         if (readlane_d(a_l, vp.asc ? nu : 0) <= 0.0 || readlane_d(a_l, b0) >= (double)NR) continue;
         const int m_l = min((int)a_l, NR - 1);
         const double f_l = a_l - (double)m_l;
+        FT[2 * kf] = f_l;
+        FT[2 * kt + 1] = g.cone ? sj * tau_l : 1.0;   // z magnification of column `lane`
         // columns (lane = tile column u, boundaries u and u+1) with a positive row extent
         const double a_n = shift_down1(a_l);
-        const unsigned long long act = __ballot(lane < nu && (vp.asc ? a_n > a_l : a_l > a_n));
+        const unsigned act = (unsigned)__ballot(lane < nu && (vp.asc ? a_n > a_l : a_l > a_n));
+        // the same bits in walk order: bit k = the column crossed at step k
+        const unsigned actk = vp.asc ? act : __builtin_bitreverse32(act) >> (32 - nu);
         const double inv_sj = invs[j];
         const T qj = (T)(g.cone ? inv_sj * inv_sj : fabs(inv_sj));
-        const double t_l = g.cone ? sj * tau_l : 1.0;     // z magnification of column `lane`
-        const CTLAB_GMEM T* __restrict__ base = cz + j * slab_off + zl;
         int m = __builtin_amdgcn_readlane(m_l, b0);
-        // rows m, m+1, m+2 in flight (row indices clamped to the last row, as
-        // 32-bit scalars: there is no scalar 64-bit compare)
-        int r2 = min(m + 2, NR - 1);
-        T c0 = base[(long)m * row_off];
-        T c1 = base[(long)min(m + 1, NR - 1) * row_off];
-        T c2 = base[(long)r2 * row_off];
+        // rows m and m+1 in flight; rp: next row to read (uniform)
+        const CTLAB_GMEM T* __restrict__ rp = cz + j * slab_off + (long)m * row_off;
+        auto row = [zoff](const CTLAB_GMEM T* r) {
+            return *(const CTLAB_GMEM T*)((const CTLAB_GMEM char*)r + zoff);
+        };
+        T c0 = row(rp);
+        T c1 = row(rp + row_off);
+        rp += 2 * row_off;
         T S = T(0);
         T Eprev = (T)readlane_d(f_l, b0) * c0;
-        // constant trip count (steps past the tile's last column repeat its
-        // boundary and add nothing), so that the loop unrolls and acc[]
-        // stays in registers
+        // every tile runs NU steps: in a partial tile (nu < NU) the steps past
+        // nu stay on the last boundary and have no active column
 #pragma unroll
-        for (int q = 1; q <= NU; ++q) {
-            const int qc = min(q, nu);
-            const int bq = vp.asc ? qc : nu - qc;          // boundary reached at this step
+        for (int k = 0; k < NU; ++k) {
+            const int bq = vp.asc ? min(k + 1, nu) : max(nu - 1 - k, 0);   // boundary reached at this step
             const int mq = __builtin_amdgcn_readlane(m_l, bq);
             while (m < mq) {
                 S += c0;
                 ++m;
                 c0 = c1;
-                c1 = c2;
-                r2 = min(r2 + 1, NR - 1);
-                c2 = base[(long)r2 * row_off];
+                c1 = row(rp);
+                rp += row_off;
             }
-            const T E = S + (T)readlane_d(f_l, bq) * c0;
+            const double2 ft = *reinterpret_cast<const double2*>(FT + 2 * k);
+            const T E = S + (T)ft.x * c0;
             const T H = E - Eprev;
             Eprev = E;
-            const int uq = vp.asc ? qc - 1 : nu - qc;      // tile column crossed at this step
-            if (q <= nu && ((act >> uq) & 1ull)) {
-                const double t = readlane_d(t_l, uq);
-                T* Gq = G + (q - 1) * kGStride;
+            if ((actk >> k) & 1u) {
+                T* Gq = G + k * kGStride;
                 Gq[lane + 1] = H;
-                const T Db = lds_interp(Gq, vp.beta0 + t * zb, ZN);
-                const T Dn = shift_down1(Db);   // lane 63: 0, its top boundary is deferred
-                acc[q - 1] += (Dn - Db) * qj;
+                const T Db = lds_interp(Gq, fma(ft.y, zb, vp.beta0), ZN);
+                if constexpr (kSumThenDiff) {
+                    acc[k] += Db * qj;
+                } else {
+                    const T Dn = shift_down1(Db);   // lane 63: 0, its top boundary is deferred
+                    acc[k] += (Dn - Db) * qj;
+                }
             }
         }
         if (defer_top) {
             // lane s < nu: top boundary of column step s (its table is region s)
-            const int us = vp.asc ? lane : nu - 1 - lane;
-            if (lane < nu && ((act >> us) & 1ull)) {
+            if ((actk >> lane) & 1u) {
                 const double t = g.cone ? sj * tau_w : 1.0;
                 acc_top += lds_interp(G + lane * kGStride, vp.beta0 + t * ztop, ZN) * qj;
             }
         }
+    }
+    if constexpr (kSumThenDiff) {
+        // row v = boundary v+1 - boundary v; lane 63 gets 0 - boundary 63
+#pragma unroll
+        for (int q = 0; q < NU; ++q) acc[q] = shift_down1(acc[q]) - acc[q];
     }
     if (defer_top) {
 #pragma unroll

@@ -45,13 +45,14 @@ zprefix_kernel(const float* __restrict__ vol, T* __restrict__ cz,
 // each wavefront loops over all slabs of its view.
 template <typename T, int NU>
 __global__ void __launch_bounds__(256)
-dd_proj_walk(DevGeom g, const T* __restrict__ cz, float* __restrict__ prj)
+__attribute__((amdgpu_waves_per_eu(8))) dd_proj_walk(DevGeom g, const T* __restrict__ cz, float* __restrict__ prj)
 {
     __shared__ T sG[4][NU * kGStride];
+    __shared__ __align__(16) double sFT[4][2 * (NU + 1)];
     const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int p = blockIdx.y * 4 + wv;
     if (p >= g.PN) return;
-    proj_walk_wave<T, NU>(g, gmem(cz), gmem(prj), p, blockIdx.x, threadIdx.x & 63, sG[wv]);
+    proj_walk_wave<T, NU>(g, gmem(cz), gmem(prj), p, blockIdx.x, threadIdx.x & 63, sG[wv], sFT[wv]);
 }
 
 // One wavefront per (view, column): v-prefix of y*L for the backprojector.

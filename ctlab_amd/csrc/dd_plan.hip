@@ -274,7 +274,6 @@ void free_plan(ctlab_plan* P)
 static int alloc_workspace(ctlab_plan* P)
 {
     const size_t es = P->sat_bits == 32 ? 4 : 8;
-    const size_t nvox = (size_t)P->in.XN * P->in.YN * P->in.ZN;
     const size_t nprj = (size_t)P->in.PN * P->in.DNU * P->in.DNV;
     if (!P->ws_shared) {
         if (P->d_cz) hipFree(P->d_cz);
@@ -282,7 +281,7 @@ static int alloc_workspace(ctlab_plan* P)
     }
     P->d_cz = P->d_P = nullptr;
     P->ws_shared = false;
-    if (hip_check(hipMalloc(&P->d_cz, nvox * es), "hipMalloc(integral image)")) return -1;
+    if (hip_check(hipMalloc(&P->d_cz, cz_elems(P->in) * es), "hipMalloc(integral image)")) return -1;
     if (hip_check(hipMalloc(&P->d_P, nprj * es), "hipMalloc(projection prefix)")) return -1;
     P->ws_elem = es;
     return 0;
@@ -605,9 +604,8 @@ int ctlab_plan_views(const ctlab_plan* plan, int* n_yslab, int* n_xslab)
 long long ctlab_plan_workspace_bytes(const ctlab_plan* plan)
 {
     if (!plan) return -1;
-    const size_t nvox = (size_t)plan->in.XN * plan->in.YN * plan->in.ZN;
     const size_t nprj = (size_t)plan->in.PN * plan->in.DNU * plan->in.DNV;
-    return (long long)((nvox + nprj) * plan->ws_elem);
+    return (long long)((cz_elems(plan->in) + nprj) * plan->ws_elem);
 }
 
 void DD3Proj_gpu(float x0, float y0, float z0, int DNU, int DNV, float* xds, float* yds, float* zds,

@@ -19,6 +19,14 @@ struct GeomIn {
     int cone;
 };
 
+// integral-image elements: the volume plus two rows of padding for the
+// projector's read-ahead past a slab's last row (an x-slab row is XN*ZN
+// elements apart, a y-slab row ZN)
+inline size_t cz_elems(const GeomIn& in)
+{
+    return (size_t)in.XN * in.YN * in.ZN + 2 * ((size_t)in.XN + 1) * in.ZN;
+}
+
 
 template <typename T>
 hipError_t launch_proj(const DevGeom& g, const float* d_vol, void* d_cz, float* d_prj, hipStream_t s);
