@@ -41,6 +41,21 @@ __device__ __forceinline__ float shift_down1(float v)
     return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x130, 0xf, 0xf, true));
 }
 
+// lane i receives lane i+1's value; lane 63, which has no source lane, keeps
+// `fill` (DPP wave_shl:1 without bound_ctrl leaves that lane unwritten)
+__device__ __forceinline__ double shift_down1_fill(double v, double fill)
+{
+    const long long b = __double_as_longlong(v), f = __double_as_longlong(fill);
+    const int lo = __builtin_amdgcn_update_dpp((int)(f & 0xffffffffLL), (int)(b & 0xffffffffLL), 0x130, 0xf, 0xf, false);
+    const int hi = __builtin_amdgcn_update_dpp((int)(f >> 32), (int)(b >> 32), 0x130, 0xf, 0xf, false);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+
+__device__ __forceinline__ float shift_down1_fill(float v, float fill)
+{
+    return __int_as_float(__builtin_amdgcn_update_dpp(__float_as_int(fill), __float_as_int(v), 0x130, 0xf, 0xf, false));
+}
+
 // clamp i to [0, hi] in one VALU instruction (hi uniform)
 __device__ __forceinline__ int clamp_idx(int i, int hi)
 {
@@ -281,33 +296,31 @@ __device__ __forceinline__ void prjprefix_wave(const DevGeom& g, const CTLAB_GME
 // the upper one from lane z+1 (DPP shift).  Lane 63 of a 64-slice volume has no
 // neighbour: its upper boundary (z = 64) mostly projects off the detector,
 // where the prefix is the column total (above) or 0 (below), and is
-// interpolated only when it lands on the detector.  top_total: the view's top
-// boundary is above the detector for every slab and column (decided on the host).
+// interpolated only when it lands on the detector; the DPP shift leaves lane 63
+// its own fill value.  top_total: the view's top boundary is above the detector
+// for every slab and column (decided on the host).  The differences follow the
+// detector row axis; the caller orients them along z (sign of the row pitch).
 // lzb = (lane - beta0) / sj (per view and slab), zoff = zhat0 / dzh: the lower
 // boundary of voxel z = lane maps to the fractional detector row esdz * lzb - zoff.
 template <typename T>
 __device__ __forceinline__ T column_delta(const DevGeom& g, T* G, T Pu, double esdz, double lzb, double zoff,
-                                          double inv_sj, double beta0, int top_total, T sgn, int lane)
+                                          double inv_sj, double beta0, int top_total, int lane)
 {
     const int DNV = g.DNV;
     G[lane + 1] = Pu;
     if (!g.cone)   // multi-slice fan beam: boundaries are the integer rows
         return lds_interp(G, (double)(lane + 1), DNV) - lds_interp(G, (double)lane, DNV);
     const T Q = lds_interp(G, esdz * lzb - zoff, DNV);
-    T Qn = shift_down1(Q);
-    if (g.ZN == 64) {
-        T top;
-        if (top_total) {
-            top = readlane_t(Pu, DNV - 1);
-        } else {
-            const double ntop = (64.0 - beta0) * inv_sj * esdz - zoff;
-            if (ntop >= (double)DNV) top = readlane_t(Pu, DNV - 1);
-            else if (ntop <= 0.0) top = T(0);
-            else top = lds_interp(G, ntop, DNV);
-        }
-        if (lane == 63) Qn = top;
+    if (g.ZN != 64) return shift_down1(Q) - Q;
+    // lane 63: upper boundary z = 64; G[DNV] is the column total
+    T top;
+    if (top_total) {
+        top = G[DNV];
+    } else {
+        const double ntop = (64.0 - beta0) * inv_sj * esdz - zoff;
+        top = ntop >= (double)DNV ? G[DNV] : ntop <= 0.0 ? T(0) : lds_interp(G, ntop, DNV);
     }
-    return (Qn - Q) * sgn;
+    return shift_down1_fill(Q, top) - Q;
 }
 
 // Column record in walk order (increasing row coordinate a): slopes of the
@@ -374,26 +387,25 @@ __device__ __forceinline__ void back_walk_accum(const DevGeom& g, const CTLAB_GM
         if (alo >= (double)(r0 + nm)) continue;
         double ahi = vp.rbase + rs * r.khi;
         double iw = inv_rs * r.invdk;
-        // first integer row boundary at or above the column's upper edge
-        int mend = __builtin_amdgcn_readfirstlane((int)ceil(fmin(fmax(ahi, -1.0e9), 1.0e9)));
         T Pr0 = pq[0];
         if (c + 1 < DNU) { rq += 1; pq += pstep; }
         ColRec rn = rq[0];
         T Pr1 = pq[0];
-        T D = column_delta(g, G, Pr0, r.esdz, lzb, zoff, inv_sj, vp.beta0, vp.top_total, sgn, lane);
+        T D = column_delta(g, G, Pr0, r.esdz, lzb, zoff, inv_sj, vp.beta0, vp.top_total, lane);
         T Tsum = T(0), Wprev = T(0);
         // not unrolled (each copy would carry its own column walk)
         for (int i = 0; i <= nm; ++i) {
             const int mi = r0 + i;
             const double mp = (double)mi;
-            while (mi >= mend && c < DNU - 1) {
+            // the column ends at or below row boundary mi (mi >= ahi: for an
+            // integer mi the same as mi >= ceil(ahi))
+            while (mp >= ahi && c < DNU - 1) {
                 Tsum += D;
                 ++c;
                 alo = ahi;
                 ahi = vp.rbase + rs * rn.khi;
-                mend = __builtin_amdgcn_readfirstlane((int)ceil(fmin(fmax(ahi, -1.0e9), 1.0e9)));
                 iw = inv_rs * rn.invdk;
-                D = column_delta(g, G, Pr1, rn.esdz, lzb, zoff, inv_sj, vp.beta0, vp.top_total, sgn, lane);
+                D = column_delta(g, G, Pr1, rn.esdz, lzb, zoff, inv_sj, vp.beta0, vp.top_total, lane);
                 if (c + 1 < DNU) { rq += 1; pq += pstep; }
                 rn = rq[0];
                 Pr1 = pq[0];
@@ -404,8 +416,10 @@ __device__ __forceinline__ void back_walk_accum(const DevGeom& g, const CTLAB_GM
             Wprev = W;
         }
     }
+    // the z-differences above are taken along the detector row axis; sgn
+    // orients them along z
 #pragma unroll
-    for (int i = 0; i < NM; ++i) acc[i] = (T)A[i * 64 + lane];
+    for (int i = 0; i < NM; ++i) acc[i] = (T)A[i * 64 + lane] * sgn;
 }
 
 // Writes one backprojected tile (lane = z): for voxel row i of the tile,

@@ -68,6 +68,27 @@ def test_back_parity_cone(cuda, arc):
     assert e32 <= TOL32
 
 
+@pytest.mark.parametrize("row_size", [1.6, 2.5, 4.0])
+def test_64_slices_and_rows(cuda, row_size):
+    """ZN = DNV = 64: voxel boundary z = 64 and detector boundary v = 64 have no
+    lane of their own (the projector defers the top row's upper boundary, the
+    backprojector fills lane 63's upper boundary).  With 1.6 mm rows the volume
+    top projects past the detector in every view, with 4.0 mm onto it, with
+    2.5 mm past it from the slabs near the source and onto it from the far ones."""
+    g = _cone(XN=24, YN=20, ZN=64, dz=1.5, PN=24, DNU=48, DNV=64, row_size=row_size, ang0=0.2)
+    vol = _phantom(g, 31)
+    ref = O.proj(g, vol)
+    e_p = rel_l2(_run(g, vol=vol), ref)
+    e_p32 = rel_l2(_run(g, vol=vol, bits=32), ref)
+    prj = np.random.default_rng(32).random(g.nprj()).astype(np.float32)
+    refb = O.back(g, prj)
+    e_b = rel_l2(_run(g, prj=prj), refb)
+    e_b32 = rel_l2(_run(g, prj=prj, bits=32), refb)
+    print(f"64 slices, row {row_size}: proj {e_p:.2e} (fp32-SAT {e_p32:.2e}), back {e_b:.2e} (fp32-SAT {e_b32:.2e})")
+    assert e_p <= TOL64 and e_b <= TOL64
+    assert e_p32 <= TOL32 and e_b32 <= TOL32
+
+
 @pytest.mark.parametrize("nu,nm", [(8, 16), (16, 8), (12, 12), (16, 16)])
 def test_tile_variants(cuda, nu, nm):
     import torch
