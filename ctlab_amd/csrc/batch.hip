@@ -24,6 +24,8 @@
 
 #include "../../include/ctlab_mi355x.h"
 #include "dd_device.hpp"
+
+#include <type_traits>
 #include "plan.hpp"
 #include "recon.hpp"
 #include "status.hpp"
@@ -162,7 +164,7 @@ __global__ void __launch_bounds__(256) bk_back(const BatchProb* __restrict__ pb,
 {
     constexpr int yslab = YSLAB;
     __shared__ T sG[4][72];
-    __shared__ typename BackAcc<T, NM>::type sA[4][NM * 64];
+    __shared__ typename BackAcc<T, NM>::type sA[4][(NM + 1) * 64];
     const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int lane = threadIdx.x & 63;
     drain<1>(q, [&](int k, int it) {
@@ -171,8 +173,16 @@ __global__ void __launch_bounds__(256) bk_back(const BatchProb* __restrict__ pb,
         const int NS = yslab ? g.YN : g.XN, NR = yslab ? g.XN : g.YN;
         const int j = it % NS, rt = it / NS;
         T acc[NM];
-        back_walk_accum<T, NM>(g, (const CTLAB_GMEM T*)b.P, yslab ? b.views_y : b.views_x, yslab ? b.ny : b.nx, NR, j, rt,
-                               b.rec, b.start, b.SJ, b.SR, lane, sG[wv], sA[wv], acc);
+        auto accum = [&](auto zm) {
+            back_walk_accum<T, NM, decltype(zm)::value>(g, (const CTLAB_GMEM T*)b.P, yslab ? b.views_y : b.views_x,
+                                                        yslab ? b.ny : b.nx, NR, j, rt, b.rec, b.start, b.SJ, b.SR,
+                                                        lane, sG[wv], sA[wv], acc);
+        };
+        switch (zmode_of(g)) {
+        case kZFan: accum(std::integral_constant<int, kZFan>()); break;
+        case kZCone: accum(std::integral_constant<int, kZCone>()); break;
+        default: accum(std::integral_constant<int, kZCone64>()); break;
+        }
         CTLAB_GMEM float* __restrict__ vol = b.vol;
         if constexpr (YSLAB) {
             back_store_tile<T, NM>(g, acc, 1, NR, j, rt, lane, [&](long idx, float v) { vol[idx] = v; });

@@ -287,6 +287,11 @@ __device__ __forceinline__ void prjprefix_wave(const DevGeom& g, const CTLAB_GME
     if (lane < DNV) P[idx * DNV + lane] = r;
 }
 
+// z-mode of the backprojector's column step
+enum ZMode { kZFan = 0, kZCone = 1, kZCone64 = 2 };   // multi-slice fan; cone, ZN < 64; cone, ZN == 64
+
+__host__ __device__ inline int zmode_of(const DevGeom& g) { return !g.cone ? kZFan : g.ZN == 64 ? kZCone64 : kZCone; }
+
 // Adjoint z-part for one detector column: sum_v y*L * (fraction of cell v
 // covered by the voxel z-interval [lane, lane+1] mapped onto the detector rows),
 // read from the column's v-prefix Pu (lane v holds prefix v+1) through LDS.
@@ -300,32 +305,53 @@ __device__ __forceinline__ void prjprefix_wave(const DevGeom& g, const CTLAB_GME
 // its own fill value.  top_total: the view's top boundary is above the detector
 // for every slab and column (decided on the host).  The differences follow the
 // detector row axis; the caller orients them along z (sign of the row pitch).
-// lzb = (lane - beta0) / sj (per view and slab), zoff = zhat0 / dzh: the lower
-// boundary of voxel z = lane maps to the fractional detector row esdz * lzb - zoff.
-template <typename T>
+template <int ZM, typename T>
 __device__ __forceinline__ T column_delta(const DevGeom& g, T* G, T Pu, double esdz, double lzb, double zoff,
                                           double inv_sj, double beta0, int top_total, int lane)
 {
     const int DNV = g.DNV;
     G[lane + 1] = Pu;
-    if (!g.cone)   // multi-slice fan beam: boundaries are the integer rows
+    if constexpr (ZM == kZFan) {   // multi-slice fan beam: boundaries are the integer rows
         return lds_interp(G, (double)(lane + 1), DNV) - lds_interp(G, (double)lane, DNV);
-    const T Q = lds_interp(G, esdz * lzb - zoff, DNV);
-    if (g.ZN != 64) return shift_down1(Q) - Q;
-    // lane 63: upper boundary z = 64; G[DNV] is the column total
-    T top;
-    if (top_total) {
-        top = G[DNV];
     } else {
-        const double ntop = (64.0 - beta0) * inv_sj * esdz - zoff;
-        top = ntop >= (double)DNV ? G[DNV] : ntop <= 0.0 ? T(0) : lds_interp(G, ntop, DNV);
+        const T Q = lds_interp(G, esdz * lzb - zoff, DNV);
+        if constexpr (ZM == kZCone) {
+            return shift_down1(Q) - Q;
+        } else {
+            T top;   // G[DNV] is the column total
+            if (top_total) {
+                top = G[DNV];
+            } else {
+                const double ntop = (64.0 - beta0) * inv_sj * esdz - zoff;
+                top = ntop >= (double)DNV ? G[DNV] : ntop <= 0.0 ? T(0) : lds_interp(G, ntop, DNV);
+            }
+            return shift_down1_fill(Q, top) - Q;
+        }
     }
-    return shift_down1_fill(Q, top) - Q;
 }
 
 // Column record in walk order (increasing row coordinate a): slopes of the
-// column's lower and upper boundary rays, 1/|slope step|, and esdz.
+// column's lower and upper boundary rays, 1/|slope step|, and esdz.  Each view
+// holds DNU records and a sentinel (upper slope NaN, 1/|slope step| and esdz
+// 0), and one spare record follows the last view.
 struct ColRec { double klo, khi, invdk, esdz; };
+
+// buffer-resource word 3 for raw (untyped, unswizzled) buffer access on gfx9
+constexpr int kBufWord3 = 0x00020000;
+
+// raw buffer load of one element at byte offset off; an offset at or past the
+// resource's size reads 0
+template <typename T> __device__ __forceinline__ T buf_load(__amdgpu_buffer_rsrc_t r, unsigned off);
+template <> __device__ __forceinline__ double buf_load<double>(__amdgpu_buffer_rsrc_t r, unsigned off)
+{
+    typedef unsigned int u2 __attribute__((ext_vector_type(2)));
+    const u2 b = __builtin_amdgcn_raw_buffer_load_b64(r, off, 0, 0);
+    return __longlong_as_double(((long long)b.y << 32) | b.x);
+}
+template <> __device__ __forceinline__ float buf_load<float>(__amdgpu_buffer_rsrc_t r, unsigned off)
+{
+    return __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(r, off, 0, 0));
+}
 
 // Backprojector walk of one (slab j, tile rt of NM rows) over the views of one
 // slab orientation; lane = z.  acc[i] (zeroed here) receives voxel row
@@ -336,7 +362,9 @@ struct ColRec { double klo, khi, invdk, esdz; };
 template <typename T, int NM> struct BackAcc { typedef T type; };
 template <> struct BackAcc<double, 16> { typedef float type; };
 
-template <typename T, int NM>
+// A: (NM + 1) * 64 accumulators of LDS; row i lives at A[(i + 1) * 64 + lane],
+// row -1 is a spare that takes the walk's first (unpaired) row boundary.
+template <typename T, int NM, int ZM>
 __device__ __forceinline__ void back_walk_accum(const DevGeom& g, const CTLAB_GMEM T* __restrict__ P,
                                                 const CTLAB_GMEM int* __restrict__ views, int nviews, int NR, int j,
                                                 int rt, const CTLAB_GMEM ColRec* __restrict__ rec,
@@ -344,15 +372,17 @@ __device__ __forceinline__ void back_walk_accum(const DevGeom& g, const CTLAB_GM
                                                 int SJ, int SR, int lane, T* G,
                                                 typename BackAcc<T, NM>::type* A, T (&acc)[NM])
 {
+    typedef typename BackAcc<T, NM>::type AccT;
     if (lane == 0) G[0] = T(0);
-    // row accumulators live in LDS (A[i*64 + lane]): the row index varies at run
-    // time, and indexing registers with it costs four scalar-set / move pairs
-    // per FP64 update
+    // row accumulators live in LDS: the row index varies at run time, and
+    // indexing registers with it costs four scalar-set / move pairs per FP64
+    // update
+    A += 64;
 #pragma unroll
     for (int i = 0; i < NM; ++i) A[i * 64 + lane] = 0;
     const int r0 = rt * NM;
     const int nm = min(NM, NR - r0);
-    const int ZN = g.ZN, DNU = g.DNU, DNV = g.DNV;
+    const int DNU = g.DNU, DNV = g.DNV;
     const int vl = min(lane, DNV - 1);
     const double inv_dx = 1.0 / g.dx;
     const long srow = (long)j * SR + (r0 >> 3);
@@ -375,44 +405,48 @@ __device__ __forceinline__ void back_walk_accum(const DevGeom& g, const CTLAB_GM
         const double inv_sj = g.invs[(long)p * g.sjs + j];
         const double inv_rs = fabs(g.dx * inv_sj);
         const double lzb = ((double)lane - vp.beta0) * inv_sj;
-        // walk-order column c: its record rp[0] and v-prefix row pp[0]; the next
-        // column's (rq, pq) are loaded one column ahead; pointers step instead of
-        // being recomputed (the scalar unit is as busy as the vector unit here)
-        int c = st;
-        const CTLAB_GMEM ColRec* __restrict__ rq = rec + (long)p * DNU + c;
-        const long pstep = vp.asc ? DNV : -DNV;
-        const CTLAB_GMEM T* __restrict__ pq = P + (long)p * DNU * DNV + vl + (long)(vp.asc ? c : DNU - 1 - c) * DNV;
+        // walk-order column c = st: its record and v-prefix row; the next
+        // column's are loaded one column ahead.  The view's v-prefix rows are
+        // read through a buffer resource, so the step past either end of the
+        // view (the sentinel column) reads 0 instead of needing a clamp.
+        const CTLAB_GMEM ColRec* __restrict__ rq = rec + (long)p * (DNU + 1) + st;
+        const __amdgpu_buffer_rsrc_t prs = __builtin_amdgcn_make_buffer_rsrc(
+            (void*)(P + (long)p * DNU * DNV), (short)0, DNU * DNV * (int)sizeof(T), kBufWord3);
+        unsigned voff = (unsigned)((vp.asc ? st : DNU - 1 - st) * DNV + vl) * (unsigned)sizeof(T);
+        const unsigned vstep = (unsigned)((vp.asc ? DNV : -DNV) * (int)sizeof(T));
         ColRec r = rq[0];
         double alo = vp.rbase + rs * r.klo;
         if (alo >= (double)(r0 + nm)) continue;
         double ahi = vp.rbase + rs * r.khi;
         double iw = inv_rs * r.invdk;
-        T Pr0 = pq[0];
-        if (c + 1 < DNU) { rq += 1; pq += pstep; }
+        const T Pr0 = buf_load<T>(prs, voff);
+        rq += 1;
+        voff += vstep;
         ColRec rn = rq[0];
-        T Pr1 = pq[0];
-        T D = column_delta(g, G, Pr0, r.esdz, lzb, zoff, inv_sj, vp.beta0, vp.top_total, lane);
+        T Pr1 = buf_load<T>(prs, voff);
+        T D = column_delta<ZM>(g, G, Pr0, r.esdz, lzb, zoff, inv_sj, vp.beta0, vp.top_total, lane);
         T Tsum = T(0), Wprev = T(0);
         // not unrolled (each copy would carry its own column walk)
         for (int i = 0; i <= nm; ++i) {
-            const int mi = r0 + i;
-            const double mp = (double)mi;
-            // the column ends at or below row boundary mi (mi >= ahi: for an
-            // integer mi the same as mi >= ceil(ahi))
-            while (mp >= ahi && c < DNU - 1) {
+            const double mp = (double)(r0 + i);
+            // advance while the column ends at or below row boundary r0 + i
+            // (for an integer boundary the same as r0 + i >= ceil(ahi)); the
+            // sentinel's NaN upper edge stops the walk after the last column,
+            // whose term has then moved into Tsum while the sentinel adds 0
+            while (mp >= ahi) {
                 Tsum += D;
-                ++c;
                 alo = ahi;
                 ahi = vp.rbase + rs * rn.khi;
                 iw = inv_rs * rn.invdk;
-                D = column_delta(g, G, Pr1, rn.esdz, lzb, zoff, inv_sj, vp.beta0, vp.top_total, lane);
-                if (c + 1 < DNU) { rq += 1; pq += pstep; }
+                D = column_delta<ZM>(g, G, Pr1, rn.esdz, lzb, zoff, inv_sj, vp.beta0, vp.top_total, lane);
+                rq += 1;
+                voff += vstep;
                 rn = rq[0];
-                Pr1 = pq[0];
+                Pr1 = buf_load<T>(prs, voff);
             }
             const double fr = fmin(fmax((mp - alo) * iw, 0.0), 1.0);
             const T W = Tsum + (T)fr * D;
-            if (i > 0) A[(i - 1) * 64 + lane] += (typename BackAcc<T, NM>::type)(W - Wprev);
+            A[(i - 1) * 64 + lane] += (AccT)(W - Wprev);
             Wprev = W;
         }
     }
@@ -444,7 +478,7 @@ __device__ __forceinline__ void back_store_tile(const DevGeom& g, const T (&acc)
 
 // Backprojection of one (slab j, row tile rt) over the listed views of one
 // slab orientation; overwrites (accumulate = 0) or adds to vol.
-template <typename T, int NM>
+template <typename T, int NM, int ZM>
 __device__ __forceinline__ void back_walk_wave(const DevGeom& g, const CTLAB_GMEM T* __restrict__ P,
                                                CTLAB_GMEM float* __restrict__ vol, const CTLAB_GMEM int* __restrict__ views,
                                                int nviews, int yslab, int accumulate, int NR, int j, int rt,
@@ -453,7 +487,7 @@ __device__ __forceinline__ void back_walk_wave(const DevGeom& g, const CTLAB_GME
                                                T* G, typename BackAcc<T, NM>::type* A)
 {
     T acc[NM];
-    back_walk_accum<T, NM>(g, P, views, nviews, NR, j, rt, rec, start, SJ, SR, lane, G, A, acc);
+    back_walk_accum<T, NM, ZM>(g, P, views, nviews, NR, j, rt, rec, start, SJ, SR, lane, G, A, acc);
     back_store_tile<T, NM>(g, acc, yslab, NR, j, rt, lane, [&](long idx, float v) {
         vol[idx] = accumulate ? vol[idx] + v : v;
     });

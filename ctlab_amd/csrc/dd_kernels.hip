@@ -26,6 +26,8 @@
 // slab orientation, so no atomics are needed.
 #include "dd_device.hpp"
 
+#include <type_traits>
+
 namespace ctlab {
 
 // One wavefront per voxel column (lane = z; ZN <= 64): coalesced read of the
@@ -81,7 +83,7 @@ back_start_kernel(DevGeom g, const ColRec* __restrict__ rec, unsigned short* __r
     const double r0 = 8.0 * r8;
     if (j >= vp.NS || r0 >= vp.NR) { start[idx] = 0xFFFF; return; }
     const double rs = (vp.s0 + j * g.dx) / g.dx;
-    const ColRec* R = rec + (long)p * DNU;
+    const ColRec* R = rec + (long)p * (DNU + 1);
     auto A = [&](int q) { return vp.rbase + rs * (q < DNU ? R[q].klo : R[DNU - 1].khi); };
     if (A(DNU) <= r0) { start[idx] = 0xFFFF; return; }
     int lo = 1, hi = DNU;
@@ -110,7 +112,7 @@ dd_back_walk(DevGeom g, const T* __restrict__ P, float* __restrict__ vol_y, floa
              int SJ, int SR)
 {
     __shared__ T sG[4][72];
-    __shared__ typename BackAcc<T, NM>::type sA[4][NM * 64];
+    __shared__ typename BackAcc<T, NM>::type sA[4][(NM + 1) * 64];
     const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int ys = blockIdx.x < (unsigned)nby;
     const int b = ys ? blockIdx.x : blockIdx.x - nby;
@@ -118,8 +120,16 @@ dd_back_walk(DevGeom g, const T* __restrict__ P, float* __restrict__ vol_y, floa
     const int rt = b % ntr;
     const int j = (b / ntr) * 4 + wv;
     if (j >= NS) return;
-    back_walk_wave<T, NM>(g, gmem(P), gmem(ys ? vol_y : vol_x), gmem(ys ? views_y : views_x), ys ? ny : nx, ys, 0,
-                          NR, j, rt, gmem(rec), gmem(start), SJ, SR, threadIdx.x & 63, sG[wv], sA[wv]);
+    auto walk = [&](auto zm) {
+        back_walk_wave<T, NM, decltype(zm)::value>(g, gmem(P), gmem(ys ? vol_y : vol_x), gmem(ys ? views_y : views_x),
+                                                   ys ? ny : nx, ys, 0, NR, j, rt, gmem(rec), gmem(start), SJ, SR,
+                                                   threadIdx.x & 63, sG[wv], sA[wv]);
+    };
+    switch (zmode_of(g)) {
+    case kZFan: walk(std::integral_constant<int, kZFan>()); break;
+    case kZCone: walk(std::integral_constant<int, kZCone>()); break;
+    default: walk(std::integral_constant<int, kZCone64>()); break;
+    }
 }
 
 // vol += add (n floats)

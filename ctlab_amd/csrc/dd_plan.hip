@@ -5,6 +5,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <limits>
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
@@ -150,7 +151,8 @@ static int build_tables(ctlab_plan* P)
     P->invdk.resize((size_t)PN * DNU);
     P->esdz.resize((size_t)PN * DNU);
     P->invs.assign((size_t)PN * std::max(g.XN, g.YN), 0.0);
-    P->rec.resize((size_t)PN * DNU * 4);
+    // DNU walk-order records and a sentinel per view, one spare record after the last view
+    P->rec.assign(((size_t)PN * (DNU + 1) + 1) * 4, 0.0);
     if (DNU >= 0xFFFF) {
         set_error(CTLAB_ERR_GEOMETRY, "DNU (%d) must be below 65535", DNU);
         return -1;
@@ -237,12 +239,17 @@ static int build_tables(ctlab_plan* P)
         // walk-order (increasing row coordinate) column records for the backprojector
         for (int c = 0; c < DNU; ++c) {
             const int u = V.asc ? c : DNU - 1 - c;
-            double* r = &P->rec[((size_t)p * DNU + c) * 4];
+            double* r = &P->rec[((size_t)p * (DNU + 1) + c) * 4];
             r[0] = V.asc ? kp[u] : kp[u + 1];
             r[1] = V.asc ? kp[u + 1] : kp[u];
             r[2] = P->invdk[(size_t)p * DNU + u];
             r[3] = P->esdz[(size_t)p * DNU + u];
         }
+        // sentinel: NaN slopes stop the column walk (no row boundary compares
+        // at or above NaN); zero 1/|slope step| and esdz keep its term finite
+        double* rs = &P->rec[((size_t)p * (DNU + 1) + DNU) * 4];
+        rs[0] = rs[1] = std::numeric_limits<double>::quiet_NaN();
+        rs[2] = rs[3] = 0.0;
     }
     return 0;
 }
