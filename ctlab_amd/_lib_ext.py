@@ -1,9 +1,11 @@
-"""ctypes declarations of the library's recon / learned-prox / batch entry points."""
+"""ctypes declarations of the library's recon / learned-prox / batch / host-level entry points."""
 import ctypes as C
 
 _vp = C.c_void_p
 _fp = C.c_void_p      # device pointers are passed as integers
 _dp = C.POINTER(C.c_double)
+_hf = C.POINTER(C.c_float)    # host float arrays
+_hb = C.POINTER(C.c_ubyte)    # host byte arrays
 
 
 def declare(L):
@@ -42,6 +44,18 @@ def declare(L):
         "ctlab_cnn_forward": ([_vp, _fp, _fp, _vp], i),
         "ctlab_cnn_loss_grad": ([_vp, _fp, _fp, _fp, C.POINTER(C.c_float), _vp], d),
         "ctlab_cnn_adam_step": ([_vp, f, f, f, f, _vp], i),
+        "ctlab_getVolZIdxPair": ([_hf, i, i, i, f, f, i, f, f, i, C.POINTER(i), C.POINTER(i)], i),
+        "ctlab_getPrjIdxPair": ([_hf, i, i, i, f, f, i, f, f, i, C.POINTER(i), C.POINTER(i)], i),
+        "ctlab_read_raw": ([C.c_char_p, _vp, ll, i], i),
+        "ctlab_write_raw": ([C.c_char_p, _vp, ll, i], i),
+        "ctlab_read_raw_device": ([C.c_char_p, _fp, ll, i], i),
+        "ctlab_write_raw_device": ([C.c_char_p, _fp, ll, i], i),
+        "ctlab_evalProjBack": ([_vp, i, i, i, _fp, _vp], i),
+        "ctlab_arc_detector": ([f, f, i, i, f, f, f, f, _hf, _hf, _hf], i),
+        "ctlab_OS_SART": ([_hf, _hf, _hf, _hf, _hf, _hb, C.c_char_p, i, i, i, f, f, i, i, i, f, f, f, i, i, i, f, f,
+                           f, f, f, f, i, i], i),
+        "ctlab_CG": ([_hf, _hf, _hf, _hf, _hf, _hb, C.c_char_p, i, i, i, f, f, i, i, i, f, f, f, i, i, i, f, f,
+                      f, f, f, f, i, i, i], i),
     }
     for name, (args, res) in sig.items():
         if hasattr(L, name):

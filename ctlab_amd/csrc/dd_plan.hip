@@ -480,6 +480,7 @@ const char* ctlab_error_string(int code)
     case CTLAB_ERR_GEOMETRY: return "unsupported geometry";
     case CTLAB_ERR_HIP: return "HIP runtime error";
     case CTLAB_ERR_NOMEM: return "device allocation failed";
+    case CTLAB_ERR_IO: return "file input/output failed";
     default: return "unknown error";
     }
 }

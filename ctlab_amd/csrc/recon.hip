@@ -13,6 +13,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cmath>
+#include <functional>
 #include <cstring>
 #include <vector>
 
@@ -123,6 +124,52 @@ double residual(ctlab_recon* R, const float* d_x, hipStream_t s, bool want_norm)
 }
 
 }  // namespace
+
+namespace ctlab {
+// CG on the normal equations (body of ctlab_cg); after_iter(k, d_x), when
+// set, runs after iteration k and stops the run by returning non-zero
+int cg_run(ctlab_recon* R, float* d_x, int iters, int cgm, double* h_resid, hipStream_t s,
+           const std::function<int(int, const float*)>& after_iter)
+{
+    if (!R || !d_x || iters < 0 || cgm < 0 || cgm > 4 || !R->have_data) {
+        set_error(CTLAB_ERR_ARG, "need recon with data, volume, iters >= 0 and 0 <= cgm <= 4");
+        return CTLAB_ERR_ARG;
+    }
+    DeviceGuard dg(R->device);
+    void* stream = (void*)s;
+    const long n = R->nvox;
+    float* r = R->d_vol2;
+    float* rn = R->d_vol3;
+    float* d = R->d_vol4;
+    float* w = R->d_vol;
+    // r = A^T (b - A x) = -gradient
+    if (std::isnan(ctlab_recon_gradient(R, d_x, r, stream))) return ctlab_last_error();
+    if (chk(rk_axpby(0.0f, r, -1.0f, r, n, s), "negate")) return ctlab_last_error();
+    hipMemcpyAsync(d, r, n * sizeof(float), hipMemcpyDeviceToDevice, s);
+    for (int it = 0; it < iters; ++it) {
+        if (plan_proj(R->full, d, R->d_prj, s)) return ctlab_last_error();
+        const double qq = dot(R, R->d_prj, nullptr, R->nprj, s);
+        const double rd = dot(R, r, d, n, s);
+        if (!(qq > 0)) break;
+        const float alpha = (float)(rd / qq);
+        if (chk(rk_axpby(alpha, d, 1.0f, d_x, n, s), "update1")) return ctlab_last_error();   // X + alpha D
+        if (plan_back(R->full, R->d_prj, w, s)) return ctlab_last_error();                     // tem2 = A^T A d
+        hipMemcpyAsync(rn, r, n * sizeof(float), hipMemcpyDeviceToDevice, s);
+        if (chk(rk_axpby(-alpha, w, 1.0f, rn, n, s), "update2")) return ctlab_last_error();   // R - alpha tem2
+        if (chk(rk_beta(r, rn, d, cgm, 1, n, R->d_part, R->d_red, s), "beta")) return ctlab_last_error();
+        double nd[2];
+        if (chk(hipMemcpyAsync(nd, R->d_red, 2 * sizeof(double), hipMemcpyDeviceToHost, s), "D2H") ||
+            chk(hipStreamSynchronize(s), "sync"))
+            return ctlab_last_error();
+        const float beta = nd[1] != 0.0 ? (float)(nd[0] / nd[1]) : 0.0f;
+        if (chk(rk_axpby(1.0f, rn, beta, d, n, s), "update3")) return ctlab_last_error();     // R + beta D
+        float* tmp = r; r = rn; rn = tmp;
+        if (h_resid) h_resid[it] = dot(R, r, nullptr, n, s);
+        if (after_iter && after_iter(it + 1, d_x)) return ctlab_last_error();
+    }
+    return chk(hipStreamSynchronize(s), "cg") ? ctlab_last_error() : CTLAB_OK;
+}
+}  // namespace ctlab
 
 extern "C" {
 
@@ -386,42 +433,7 @@ int ctlab_fista_tv(ctlab_recon* R, float* d_x, int iters, float tv_lambda, int t
 int ctlab_cg(ctlab_recon* R, float* d_x, int iters, int cgm, double* h_resid, void* stream)
 {
     clear_error();
-    if (!R || !d_x || iters < 0 || cgm < 0 || cgm > 4 || !R->have_data) {
-        set_error(CTLAB_ERR_ARG, "need recon with data, volume, iters >= 0 and 0 <= cgm <= 4");
-        return CTLAB_ERR_ARG;
-    }
-    DeviceGuard dg(R->device);
-    hipStream_t s = (hipStream_t)stream;
-    const long n = R->nvox;
-    float* r = R->d_vol2;
-    float* rn = R->d_vol3;
-    float* d = R->d_vol4;
-    float* w = R->d_vol;
-    // r = A^T (b - A x) = -gradient
-    if (std::isnan(ctlab_recon_gradient(R, d_x, r, stream))) return ctlab_last_error();
-    if (chk(rk_axpby(0.0f, r, -1.0f, r, n, s), "negate")) return ctlab_last_error();
-    hipMemcpyAsync(d, r, n * sizeof(float), hipMemcpyDeviceToDevice, s);
-    for (int it = 0; it < iters; ++it) {
-        if (plan_proj(R->full, d, R->d_prj, s)) return ctlab_last_error();
-        const double qq = dot(R, R->d_prj, nullptr, R->nprj, s);
-        const double rd = dot(R, r, d, n, s);
-        if (!(qq > 0)) break;
-        const float alpha = (float)(rd / qq);
-        if (chk(rk_axpby(alpha, d, 1.0f, d_x, n, s), "update1")) return ctlab_last_error();   // X + alpha D
-        if (plan_back(R->full, R->d_prj, w, s)) return ctlab_last_error();                     // tem2 = A^T A d
-        hipMemcpyAsync(rn, r, n * sizeof(float), hipMemcpyDeviceToDevice, s);
-        if (chk(rk_axpby(-alpha, w, 1.0f, rn, n, s), "update2")) return ctlab_last_error();   // R - alpha tem2
-        if (chk(rk_beta(r, rn, d, cgm, 1, n, R->d_part, R->d_red, s), "beta")) return ctlab_last_error();
-        double nd[2];
-        if (chk(hipMemcpyAsync(nd, R->d_red, 2 * sizeof(double), hipMemcpyDeviceToHost, s), "D2H") ||
-            chk(hipStreamSynchronize(s), "sync"))
-            return ctlab_last_error();
-        const float beta = nd[1] != 0.0 ? (float)(nd[0] / nd[1]) : 0.0f;
-        if (chk(rk_axpby(1.0f, rn, beta, d, n, s), "update3")) return ctlab_last_error();     // R + beta D
-        float* tmp = r; r = rn; rn = tmp;
-        if (h_resid) h_resid[it] = dot(R, r, nullptr, n, s);
-    }
-    return chk(hipStreamSynchronize(s), "cg") ? ctlab_last_error() : CTLAB_OK;
+    return ctlab::cg_run(R, d_x, iters, cgm, h_resid, (hipStream_t)stream, nullptr);
 }
 
 // ---- elementwise entry points (device pointers, stream-ordered) ---------

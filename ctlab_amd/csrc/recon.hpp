@@ -34,3 +34,12 @@ struct ctlab_recon {
     int graph_nonneg = -1;
     float* graph_x = nullptr;           // volume the captured graph updates in place
 };
+
+#include <functional>
+
+namespace ctlab {
+// CG on the normal equations (the body of ctlab_cg); after_iter(k, d_x), when
+// set, runs after iteration k (1-based) and stops the run by returning non-zero
+int cg_run(ctlab_recon* R, float* d_x, int iters, int cgm, double* h_resid, hipStream_t s,
+           const std::function<int(int, const float*)>& after_iter);
+}  // namespace ctlab

@@ -212,3 +212,29 @@ class Plan:
             self.close()
         except Exception:
             pass
+
+
+def getVolZIdxPair(zPos, PrjIdx_Start, PrjIdx_End, detCntIdxV, detStpZ, DNV, objCntIdxZ, dz, ZN):
+    """Slices [start, end) that views [PrjIdx_Start, PrjIdx_End) of a helical
+    scan can reach (src/utilities.cuh:478-499; zPos ascending; detCntIdxV and
+    detStpZ: row centre index and row pitch at the detector; objCntIdxZ: the
+    volume's centre slice index)."""
+    z = _f32(zPos)
+    s, e = C.c_int(), C.c_int()
+    rc = _lib.lib().ctlab_getVolZIdxPair(fptr(z), z.size, int(PrjIdx_Start), int(PrjIdx_End), float(detCntIdxV),
+                                         float(detStpZ), int(DNV), float(objCntIdxZ), float(dz), int(ZN),
+                                         C.byref(s), C.byref(e))
+    _lib.check(rc)
+    return s.value, e.value
+
+
+def getPrjIdxPair(zPos, ObjZIdx_Start, ObjZIdx_End, objCntIdxZ, dz, ZN, detCntIdxV, detStpZ, DNV):
+    """Views [start, end) whose detector can reach slices [ObjZIdx_Start,
+    ObjZIdx_End) (src/utilities.cuh:505-532)."""
+    z = _f32(zPos)
+    s, e = C.c_int(), C.c_int()
+    rc = _lib.lib().ctlab_getPrjIdxPair(fptr(z), z.size, int(ObjZIdx_Start), int(ObjZIdx_End), float(objCntIdxZ),
+                                        float(dz), int(ZN), float(detCntIdxV), float(detStpZ), int(DNV),
+                                        C.byref(s), C.byref(e))
+    _lib.check(rc)
+    return s.value, e.value

@@ -109,6 +109,19 @@ def circular_cone(XN, YN, ZN, dx, dz, PN, DNU, DNV, col_size, row_size, sid=541.
                         zpos=np.zeros(PN), XN=XN, YN=YN, ZN=ZN, dx=dx, dz=dz, mask=mask)
 
 
+def helical_cone(XN, YN, ZN, dx, dz, PN, DNU, DNV, col_size, row_size, pitch, turns=1.0, sid=541.0, sdd=949.0,
+                 arc=True, mask=None, ang0=0.0, col_offset=0.0, row_offset=0.0):
+    """Helical cone-beam scan: PN views over `turns` rotations while the source
+    and detector advance `pitch` mm along z per rotation (hzPos, the per-view
+    z offset of DD3Proj_gpu), centred on the volume."""
+    g = circular_cone(XN, YN, ZN, dx, dz, PN, DNU, DNV, col_size, row_size, sid, sdd, arc, mask, ang0,
+                      col_offset, row_offset)
+    t = np.arange(PN, dtype=np.float64) * (turns / PN)        # rotations elapsed at view p
+    g.angs = (ang0 + 2.0 * np.pi * t).astype(np.float32)
+    g.zpos = (pitch * (t - 0.5 * turns)).astype(np.float32)
+    return g
+
+
 def fan_multislice(XN, YN, SLN, dx, PN, DNU, col_size, sid=541.0, sdd=949.0, arc=True,
                    mask=None, ang0=0.0, col_offset=0.0):
     """Multi-slice fan beam (DD2_multiGPU): SLN independent 2-D slices."""

@@ -187,3 +187,74 @@ def tv_prox(b, lam, iters, shape, stream=None):
     work = torch.empty(7 * b.numel(), dtype=torch.float32, device=b.device)
     _lib.check(_lib.lib().ctlab_tv_prox(_p(b), _p(x), lam, iters, XN, YN, ZN, _p(work), _stream_handle(stream)))
     return x
+
+
+# ---- host-level drivers with the reference's argument lists ---------------
+
+def evalProjBack(plan: Plan, forwardProjection, forwardMethod=0, backwardMethod=0, stream=None):
+    """A*1 (forwardProjection) or A^T*1 over the plan's geometry, as a new
+    tensor (src/DD3_GPU_recon.h:21-28)."""
+    import torch
+    g = plan.g
+    shape = g.prj_shape if forwardProjection else g.vol_shape
+    out = torch.empty(shape, dtype=torch.float32, device=f"cuda:{plan.device}")
+    rc = _lib.lib().ctlab_evalProjBack(plan.h, 1 if forwardProjection else 0, int(forwardMethod),
+                                       int(backwardMethod), out.data_ptr(), _stream_handle(stream))
+    _lib.check(rc)
+    return out
+
+
+def driver_detector(sid, sdd, DNU, DNV, col_size, row_size, col_offset=0.0, row_offset=0.0):
+    """(xds, yds, zds) that OS_SART / CG build from their scan arguments (the
+    library's own construction, from float32 arguments)."""
+    xds, yds, zds = (np.zeros(DNU, np.float32), np.zeros(DNU, np.float32), np.zeros(DNV, np.float32))
+    rc = _lib.lib().ctlab_arc_detector(float(sid), float(sdd), int(DNU), int(DNV), float(col_size), float(row_size),
+                                       float(col_offset), float(row_offset), _lib.fptr(xds), _lib.fptr(yds),
+                                       _lib.fptr(zds))
+    _lib.check(rc)
+    return xds, yds, zds
+
+
+def _host_driver(name, extra, initImg, hprj, hangs, hzPos, mask, VolumeName, osNum, iterNum, outputMedRes, sid,
+                 sdd, DNU, DNV, PN, imgXCenter, imgYCenter, imgZCenter, XN, YN, ZN, dx, dz, col_size, row_size,
+                 col_offset, row_offset, forwardMethod, backwardMethod):
+    f32 = lambda a: None if a is None else np.ascontiguousarray(a, dtype=np.float32).reshape(-1)
+    hprj, hangs, hzPos, initImg = f32(hprj), f32(hangs), f32(hzPos), f32(initImg)
+    if hprj.size != PN * DNU * DNV or hangs.size != PN:
+        raise ValueError("projection or view-angle array does not match PN, DNU, DNV")
+    if initImg is not None and initImg.size != XN * YN * ZN:
+        raise ValueError("initial volume does not match XN, YN, ZN")
+    m = None if mask is None else np.ascontiguousarray(mask, dtype=np.uint8).reshape(-1)
+    out = np.zeros(XN * YN * ZN, dtype=np.float32)
+    fp = lambda a: None if a is None else _lib.fptr(a)
+    name_b = VolumeName.encode() if VolumeName else None
+    rc = getattr(_lib.lib(), name)(
+        _lib.fptr(out), fp(initImg), _lib.fptr(hprj), _lib.fptr(hangs), fp(hzPos), _lib.bptr(m), name_b, int(osNum),
+        int(iterNum), 1 if outputMedRes else 0, float(sid), float(sdd), int(DNU), int(DNV), int(PN),
+        float(imgXCenter), float(imgYCenter), float(imgZCenter), int(XN), int(YN), int(ZN), float(dx), float(dz),
+        float(col_size), float(row_size), float(col_offset), float(row_offset), int(forwardMethod),
+        int(backwardMethod), *extra)
+    _lib.check(rc)
+    return out.reshape(YN, XN, ZN)
+
+
+def OS_SART(initImg, hprj, hangs, hzPos, mask, VolumeName, osNum, iterNum, outputMedRes, sid, sdd, DNU, DNV, PN,
+            imgXCenter, imgYCenter, imgZCenter, XN, YN, ZN, dx, dz, col_size, row_size, col_offset, row_offset,
+            forwardMethod=0, backwardMethod=0):
+    """OS-SART with the argument list of src/DD3_GPU_recon.h:60-65 (host
+    arrays; returns reconImg as (YN, XN, ZN)).  Arc detector built from sid,
+    sdd, cell sizes and offsets; the result also goes to VolumeName (raw
+    float, (Z,X,Y)) and, with outputMedRes, iteration k to VolumeName_<k>."""
+    return _host_driver("ctlab_OS_SART", (), initImg, hprj, hangs, hzPos, mask, VolumeName, osNum, iterNum,
+                        outputMedRes, sid, sdd, DNU, DNV, PN, imgXCenter, imgYCenter, imgZCenter, XN, YN, ZN, dx, dz,
+                        col_size, row_size, col_offset, row_offset, forwardMethod, backwardMethod)
+
+
+def CG(initImg, hprj, hangs, hzPos, mask, VolumeName, osNum, iterNum, outputMedRes, sid, sdd, DNU, DNV, PN,
+       imgXCenter, imgYCenter, imgZCenter, XN, YN, ZN, dx, dz, col_size, row_size, col_offset, row_offset,
+       forwardMethod=0, backwardMethod=0, cgm=CG_FR):
+    """CG with the argument list of src/DD3_GPU_recon.h:98-103 (osNum is
+    accepted and unused: CG runs over all views)."""
+    return _host_driver("ctlab_CG", (int(cgm),), initImg, hprj, hangs, hzPos, mask, VolumeName, osNum, iterNum,
+                        outputMedRes, sid, sdd, DNU, DNV, PN, imgXCenter, imgYCenter, imgZCenter, XN, YN, ZN, dx, dz,
+                        col_size, row_size, col_offset, row_offset, forwardMethod, backwardMethod)

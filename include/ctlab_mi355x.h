@@ -94,7 +94,8 @@ enum {
     CTLAB_ERR_MODE = 2,      /* unsupported prjMode / method */
     CTLAB_ERR_GEOMETRY = 3,  /* unsupported geometry (source inside volume, ZN or DNV > 64, ...) */
     CTLAB_ERR_HIP = 4,       /* HIP runtime error */
-    CTLAB_ERR_NOMEM = 5      /* device allocation failed */
+    CTLAB_ERR_NOMEM = 5,     /* device allocation failed */
+    CTLAB_ERR_IO = 6         /* file could not be opened, read or written in full */
 };
 int ctlab_last_error(void);
 const char* ctlab_error_string(int code);
@@ -163,6 +164,61 @@ double ctlab_recon_data_fidelity(ctlab_recon* recon, const float* d_x, void* str
 double ctlab_recon_gradient(ctlab_recon* recon, const float* d_y, float* d_g, void* stream);
 /* largest eigenvalue of A^T A (power iteration) */
 double ctlab_recon_lipschitz(ctlab_recon* recon, int iters, void* stream);
+
+/* ---- CTLAB's host-level functions around the projector -------------------
+ * C-ABI forms of C++ functions of the reference (thrust::host_vector and
+ * std::string arguments become pointers and sizes); include/ctlab_mi355x.hpp
+ * restores the reference's own names and signatures for C++ callers. */
+
+/* src/utilities.cuh:478-499 getVolZIdxPair: slices [*objStart, *objEnd) that
+ * views [prjStart, prjEnd) of a helical scan can reach (zPos ascending; row
+ * centre index detCntIdxV, row pitch detStpZ at the detector, volume centre
+ * index objCntIdxZ), clamped to [0, ZN]. */
+int ctlab_getVolZIdxPair(const float* zPos, int PN, int prjStart, int prjEnd, float detCntIdxV, float detStpZ,
+                         int DNV, float objCntIdxZ, float dz, int ZN, int* objStart, int* objEnd);
+/* src/utilities.cuh:505-532 getPrjIdxPair: views [*prjStart, *prjEnd) whose
+ * detector can reach slices [objStart, objEnd), clamped to [0, PN]. */
+int ctlab_getPrjIdxPair(const float* zPos, int PN, int objStart, int objEnd, float objCntIdxZ, float dz, int ZN,
+                        float detCntIdxV, float detStpZ, int DNV, int* prjStart, int* prjEnd);
+
+/* src/readwrite.hpp:15-113 readData_host / writeData_host /
+ * writeFileFromDeviceToDisk: n raw native-endian elements of elemBytes (4:
+ * float, 8: double); host or device memory.  A short file is CTLAB_ERR_IO and
+ * leaves the buffer untouched. */
+int ctlab_read_raw(const char* fileName, void* h_data, long long n, int elemBytes);
+int ctlab_write_raw(const char* fileName, const void* h_data, long long n, int elemBytes);
+int ctlab_read_raw_device(const char* fileName, void* d_data, long long n, int elemBytes);
+int ctlab_write_raw_device(const char* fileName, const void* d_data, long long n, int elemBytes);
+
+/* src/DD3_GPU_recon.h:21-28 evalProjBack: A*1 (forwardProjection != 0, into
+ * a projection-sized d_out) or A^T*1 (volume-sized d_out) over a plan. */
+int ctlab_evalProjBack(ctlab_plan* plan, int forwardProjection, int forwardMethod, int backwardMethod, float* d_out,
+                       void* stream);
+
+/* src/DD3_GPU_recon.h:37-65 OS_SART and :67-103 CG, same arguments in the
+ * same order.  Geometry: source at (0, sid, 0) at view 0, equiangular arc
+ * detector centred on the source (cell u at angle (u - (DNU-1)/2 - col_offset)
+ * * col_size / sdd, row v at (v - (DNV-1)/2 - row_offset) * row_size), views
+ * hangs / hzPos.  initImg NULL starts from 0.  OS-SART: lambda 1, osNum
+ * interleaved subsets; CG: all views (osNum ignored), beta rule cgm.  Runs on
+ * the current device; reconImg receives the result, which is also written
+ * to volumeName (raw float, (Z,X,Y) order) unless volumeName is NULL or "";
+ * outputMedRes writes iteration k to volumeName_<k>.  Only method 0 (branchless)
+ * is built: other forward / backward methods give CTLAB_ERR_MODE. */
+/* the detector cell centres (view 0) that ctlab_OS_SART / ctlab_CG build
+ * from their scan arguments: xds, yds (DNU), zds (DNV) */
+int ctlab_arc_detector(float sid, float sdd, int DNU, int DNV, float col_size, float row_size, float col_offset,
+                       float row_offset, float* xds, float* yds, float* zds);
+int ctlab_OS_SART(float* reconImg, const float* initImg, const float* hprj, const float* hangs, const float* hzPos,
+                  const byte* mask, const char* volumeName, int osNum, int iterNum, int outputMedRes, float sid,
+                  float sdd, int DNU, int DNV, int PN, float imgXCenter, float imgYCenter, float imgZCenter, int XN,
+                  int YN, int ZN, float dx, float dz, float col_size, float row_size, float col_offset,
+                  float row_offset, int forwardMethod, int backwardMethod);
+int ctlab_CG(float* reconImg, const float* initImg, const float* hprj, const float* hangs, const float* hzPos,
+             const byte* mask, const char* volumeName, int osNum, int iterNum, int outputMedRes, float sid, float sdd,
+             int DNU, int DNV, int PN, float imgXCenter, float imgYCenter, float imgZCenter, int XN, int YN, int ZN,
+             float dx, float dz, float col_size, float row_size, float col_offset, float row_offset,
+             int forwardMethod, int backwardMethod, int cgm);
 
 /* ---- batched OS-SART (this library's own; the reference runs one problem per
  * call, src/DD3_GPU_recon.h:60-65) -----------------------------------------

@@ -1,0 +1,132 @@
+"""Host-level functions that need no GPU: raw file input / output
+(src/readwrite.hpp) and the helical view / slice range helpers
+(src/utilities.cuh:478-532), checked through the C ABI.
+
+The range helpers are pinned by what they are for: every slice that the
+oracle's dense system matrix shows a view range touching lies inside the slice
+range returned for it, and every view touching a slice range lies inside the
+view range returned for it."""
+import numpy as np
+import pytest
+
+import oracle as O
+from ctlab_amd import _lib, dd, rawio
+from ctlab_amd import geometry as G
+
+
+def test_raw_round_trip_float_and_double(tmp_path):
+    rng = np.random.default_rng(5)
+    for dt in (np.float32, np.float64):
+        a = rng.standard_normal((3, 4, 5)).astype(dt)
+        p = tmp_path / f"a_{np.dtype(dt).name}.raw"
+        rawio.write_raw(p, a)
+        assert p.stat().st_size == a.nbytes
+        assert np.array_equal(np.fromfile(p, dtype=dt).reshape(a.shape), a)   # plain native-endian array
+        assert np.array_equal(rawio.read_raw(p, dtype=dt, shape=a.shape), a)
+        assert np.array_equal(rawio.read_raw(p, n=7, dtype=dt), a.reshape(-1)[:7])
+
+
+def test_raw_errors_leave_buffer(tmp_path):
+    p = tmp_path / "short.raw"
+    np.arange(5, dtype=np.float32).tofile(p)
+    L = _lib.lib()
+    buf = np.full(8, -1.0, np.float32)
+    rc = L.ctlab_read_raw(str(p).encode(), buf.ctypes.data, 8, 4)
+    assert rc == 6 and L.ctlab_last_error() == 6          # CTLAB_ERR_IO
+    assert np.all(buf == -1.0)
+    assert L.ctlab_read_raw(str(tmp_path / "missing.raw").encode(), buf.ctypes.data, 1, 4) == 6
+    assert L.ctlab_read_raw(str(p).encode(), buf.ctypes.data, 1, 2) == 1     # element size
+    assert L.ctlab_write_raw(str(tmp_path / "nodir" / "x.raw").encode(), buf.ctypes.data, 1, 4) == 6
+    with pytest.raises(_lib.CtlabError):
+        rawio.read_raw(p, n=6)
+
+
+def test_rawio_cli(tmp_path, capsys):
+    v = np.arange(2 * 3 * 4, dtype=np.float64).reshape(2, 3, 4)
+    src = tmp_path / "v64.raw"
+    v.tofile(src)
+    dst = tmp_path / "v32.raw"
+    assert rawio.main(["convert", str(src), str(dst), "--from", "f64", "--to", "f32"]) == 0
+    assert np.array_equal(np.fromfile(dst, np.float32), v.reshape(-1).astype(np.float32))
+    assert rawio.main(["info", str(dst), "--shape", "2,3,4"]) == 0
+    assert "24 x f32" in capsys.readouterr().out
+    pgm = tmp_path / "s.pgm"
+    assert rawio.main(["slice", str(dst), str(pgm), "--shape", "2,3,4", "--z", "1"]) == 0
+    data = pgm.read_bytes()
+    assert data.startswith(b"P5\n3 2\n255\n") and len(data) == len(b"P5\n3 2\n255\n") + 6
+
+
+def _helix():
+    return G.helical_cone(XN=6, YN=5, ZN=12, dx=2.0, dz=2.5, PN=24, DNU=9, DNV=4, col_size=3.0, row_size=2.0,
+                          pitch=12.0, turns=2.0, sid=40.0, sdd=70.0, ang0=0.2, row_offset=0.25)
+
+
+def _touch(g):
+    """touch[p, k]: view p has a non-zero system-matrix entry in slice k (dense oracle)."""
+    A = O.dense(g).reshape(g.PN, g.DNU * g.DNV, g.YN * g.XN, g.ZN)
+    return np.abs(A).sum(axis=(1, 2)) > 0
+
+
+def _helper_args(g):
+    # row centre index and pitch at the detector; centre slice index of the volume
+    return dict(detCntIdxV=(g.DNV - 1) / 2.0 + 0.25, detStpZ=2.0, DNV=g.DNV,
+                objCntIdxZ=(g.ZN - 1) / 2.0 - g.imgZCenter / g.dz, dz=g.dz, ZN=g.ZN)
+
+
+def test_vol_z_range_contains_every_touched_slice():
+    g = _helix()
+    touch = _touch(g)
+    a = _helper_args(g)
+    informative = 0
+    for ps in range(0, g.PN, 3):
+        for pe in (ps + 1, ps + 4, g.PN):
+            if pe > g.PN:
+                continue
+            s, e = dd.getVolZIdxPair(g.zpos, ps, pe, a["detCntIdxV"], a["detStpZ"], a["DNV"], a["objCntIdxZ"],
+                                     a["dz"], a["ZN"])
+            ks = np.nonzero(touch[ps:pe].any(axis=0))[0]
+            assert 0 <= s <= e <= g.ZN
+            assert ks.size == 0 or (s <= ks.min() and ks.max() < e), (ps, pe, s, e, ks)
+            informative += (e - s) < g.ZN
+    assert informative > 5          # ranges narrower than the volume for short view ranges
+
+
+def test_prj_range_contains_every_touching_view():
+    g = _helix()
+    touch = _touch(g)
+    a = _helper_args(g)
+    informative = 0
+    for ks in range(0, g.ZN, 2):
+        for ke in (ks + 1, ks + 3):
+            if ke > g.ZN:
+                continue
+            s, e = dd.getPrjIdxPair(g.zpos, ks, ke, a["objCntIdxZ"], a["dz"], a["ZN"], a["detCntIdxV"],
+                                    a["detStpZ"], a["DNV"])
+            ps = np.nonzero(touch[:, ks:ke].any(axis=1))[0]
+            assert 0 <= s <= e <= g.PN
+            assert ps.size == 0 or (s <= ps.min() and ps.max() < e), (ks, ke, s, e, ps)
+            informative += (e - s) < g.PN
+    assert informative > 3
+
+
+def test_range_helper_argument_checks():
+    L = _lib.lib()
+    z = np.array([0.0, 1.0, 0.5], np.float32)          # not ascending
+    with pytest.raises(_lib.CtlabError):
+        dd.getPrjIdxPair(z, 0, 2, 1.0, 1.0, 4, 1.5, 1.0, 4)
+    with pytest.raises(_lib.CtlabError):
+        dd.getVolZIdxPair(np.zeros(4, np.float32), 3, 2, 1.5, 1.0, 4, 1.5, 1.0, 4)     # empty view range
+    assert L.ctlab_last_error() == 1
+
+
+def test_driver_detector_is_the_arc_detector():
+    """The OS_SART / CG drivers' detector equals geometry.arc_detector built
+    from the same float32 arguments, to float32 rounding of the trigonometry."""
+    from ctlab_amd import recon as RC
+    args = (150.0, 260.0, 72, 12, 1.3, 1.6, 0.25, -0.5)
+    got = RC.driver_detector(*args)
+    f = [float(np.float32(a)) if isinstance(a, float) else a for a in args]
+    ref = G.arc_detector(*f)
+    for a, b in zip(got, ref):
+        assert np.allclose(a, b, rtol=0, atol=2e-5 * max(1.0, float(np.abs(b).max())))
+    assert np.array_equal(got[2], ref[2])
