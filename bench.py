@@ -73,6 +73,26 @@ def cpu_baseline(g, vol, prj_gpu, back_fn, nviews):
     }, {"proj_rel_l2_vs_fp64": e_p, "back_rel_l2_vs_fp64": e_b, "views_checked": nviews}
 
 
+def target_report(h_plain, runs, target):
+    """Time to `target` dB: plain FISTA-TV against each learned run, with the
+    learned output's ||Ax-b||^2 beside the plain iterate's at the same
+    iteration, and the time each needs to plain FISTA-TV's own best PSNR."""
+    from ctlab_amd import learned as LE
+    best = max(h_plain, key=lambda x: x["psnr"])
+    tp = LE.time_to_psnr(h_plain, target)
+    out = {"target_db": target, "plain_reaches_target": tp[0] is not None, "plain_t_s": tp[0],
+           "plain_best_psnr": best["psnr"], "plain_best_iter": best["iter"], "plain_t_best_s": best["t"]}
+    for name, h in runs.items():
+        t, it, f = LE.time_to_psnr(h, target)
+        tb = LE.time_to_psnr(h, best["psnr"])
+        out[name] = {"t_s": t, "iter": it, "fidelity": f,
+                     "plain_fidelity_same_iter": h_plain[it - 1]["fidelity"] if it else None,
+                     "speedup_vs_plain": (tp[0] / t) if (tp[0] and t) else None,
+                     "t_to_plain_best_s": tb[0],
+                     "speedup_to_plain_best": (best["t"] / tb[0]) if tb[0] else None}
+    return out
+
+
 def learned_bench(args):
     """cfg5: train the learned prox on projector-generated pairs, then learned
     safeguarded FISTA vs plain FISTA-TV on a 512x512x64 phantom (120 views, I0 1e4)."""
@@ -83,19 +103,28 @@ def learned_bench(args):
     dev = torch.device("cuda:0")
     log = lambda m: print(m, file=sys.stderr, flush=True)
     tshape = (256, 256, 32)
-    # plain FISTA-TV lambda chosen on a held-out validation phantom (seed 999) at the training size
+    # plain FISTA-TV lambda chosen on a held-out validation phantom (seed 998) at the evaluation
+    # size (the best weight depends on the object size: at 256x256x32 it comes out smaller)
+    gs = LE.cone_geometry(512, 512, 64, 120)
+    ps = dd.Plan(gs)
+    rs = RC.Recon(ps, 1)
+    Ls = rs.lipschitz(30) * 1.01
+    ph_s = LE.make_problem(gs, ps, rs, 998, 1e4, dev)
+    zs = LE.covered_slices(gs)
+    sweep = {}
+    for lam in (0.03, 0.05, 0.1, 0.2, 0.3, 1.0, 3.0):
+        _, hv, _ = LE.fista_tv_run(rs, Ls, gs.vol_shape, args.iters, lam, 10, ph_s, zs, dev)
+        sweep[lam] = max(h["psnr"] for h in hv)
+        log(f"lambda {lam}: best validation PSNR {sweep[lam]:.2f} dB")
+    lam = max(sweep, key=sweep.get)
+    del ph_s, rs, ps
+    # validation of the trained step at the training size (seed 999)
     gv = LE.cone_geometry(tshape[1], tshape[0], tshape[2], 120)
     pv = dd.Plan(gv)
     rv = RC.Recon(pv, 1)
     Lt = rv.lipschitz(30) * 1.01
     ph_v = LE.make_problem(gv, pv, rv, 999, 1e4, dev)
     zv = LE.covered_slices(gv)
-    sweep = {}
-    for lam in (0.1, 0.3, 1.0, 3.0, 10.0):
-        _, hv, _ = LE.fista_tv_run(rv, Lt, gv.vol_shape, args.iters, lam, 10, ph_v, zv, dev)
-        sweep[lam] = max(h["psnr"] for h in hv)
-        log(f"lambda {lam}: best validation PSNR {sweep[lam]:.2f} dB")
-    lam = max(sweep, key=sweep.get)
     per = 4
     if args.params:
         params = np.load(args.params).astype(np.float32)
@@ -115,6 +144,8 @@ def learned_bench(args):
     _, hvl, _ = LE.fista_tv_run(rv, Lt, gv.vol_shape, args.iters, lam, 10, ph_v, zv, dev, learned=net_v)
     _, hvo, _ = LE.fista_tv_run(rv, Lt, gv.vol_shape, args.iters, lam, 10, ph_v, zv, dev, learned=net_v,
                                 learned_every=5, feedback=False)
+    _, hvd, _ = LE.fista_tv_run(rv, Lt, gv.vol_shape, args.iters, lam, 10, ph_v, zv, dev,
+                                learned=LE.DataConsistent(net_v, rv, args.dc_steps), learned_every=5, feedback=False)
     del net_v
     # evaluation phantom, 512x512x64, seed 7
     ge = LE.cone_geometry(512, 512, 64, 120)
@@ -134,6 +165,12 @@ def learned_bench(args):
     # discrepancy principle with the customary factor tau = 1.1 on the data-only noise estimate
     _, h_disc, _ = LE.fista_tv_run(re_, Le, ge.vol_shape, args.iters, lam, 10, ph, ze, dev, learned=net,
                                    learned_every=5, feedback=False, noise_level=1.1 * noise)
+    # learned step with the data-consistency stage (dc_steps CG iterations after the network), strict safeguard
+    dc = LE.DataConsistent(net, re_, args.dc_steps)
+    _, h_dc5, _ = LE.fista_tv_run(re_, Le, ge.vol_shape, args.iters, lam, 10, ph, ze, dev, learned=dc,
+                                  learned_every=5, feedback=False)
+    _, h_dc10, _ = LE.fista_tv_run(re_, Le, ge.vol_shape, args.iters, lam, 10, ph, ze, dev, learned=dc,
+                                   learned_every=10, feedback=False)
     target = 40.0
     best_plain = max(h["psnr"] for h in h_plain)
     tp = LE.time_to_psnr(h_plain, target)
@@ -166,11 +203,16 @@ def learned_bench(args):
                   "shape": list(tshape), "views": 120, "I0": 1e4, "epochs": args.epochs,
                   "steps": steps, "gen_s": t_gen, "train_s": t_train, "loss_per_epoch": losses,
                   "loss_identity_map": loss_id, "params_loaded": bool(args.params)},
-        "validation_256": {"plain_best_psnr": sweep[lam], "learned_in_loop_best_psnr": max(h["psnr"] for h in hvl),
+        "validation_256": {"plain_best_psnr": max(h["psnr"] for h in
+                                                  LE.fista_tv_run(rv, Lt, gv.vol_shape, args.iters, lam, 10, ph_v,
+                                                                  zv, dev)[1]),
+                           "learned_in_loop_best_psnr": max(h["psnr"] for h in hvl),
                            "learned_in_loop_accepted": sum(h["learned"] for h in hvl),
                            "learned_output_best_psnr": max(h["psnr"] for h in hvo),
-                           "learned_output_accepted": sum(h["learned"] for h in hvo)},
-        "tv_lambda_sweep_val_psnr": {str(k): v for k, v in sweep.items()}, "tv_lambda": lam,
+                           "learned_output_accepted": sum(h["learned"] for h in hvo),
+                           "learned_dc_output_best_psnr": max(h["psnr"] for h in hvd),
+                           "learned_dc_output_accepted": sum(h["learned"] for h in hvd)},
+        "tv_lambda_sweep_val512_psnr": {str(k): v for k, v in sweep.items()}, "tv_lambda": lam,
         "eval": {"shape": [512, 512, 64], "views": 120, "I0": 1e4, "iters": args.iters,
                  "psnr_slices": [ze.start, ze.stop], "fidelity_of_phantom": f_truth,
                  "noise_energy_estimate": noise},
@@ -179,11 +221,17 @@ def learned_bench(args):
         "learned_in_loop_every5": summary(h_every),
         "learned_output_every5": summary(h_out),
         "learned_output_every5_discrepancy": summary(h_disc),
+        "learned_dc_output_every5": summary(h_dc5),
+        "learned_dc_output_every10": summary(h_dc10),
+        "dc_steps": args.dc_steps,
+        "target_40dB": target_report(h_plain, {"learned_dc_output_every10": h_dc10,
+                                               "learned_dc_output_every5": h_dc5}, target),
         "ref_psnr_db": ref_db,
         "cnn_forward": {"shape": [512, 512, 64], "s": t_fwd, "tflops": rate / 1e12,
                         "mfma_util_vs_bf16_dense_peak": rate / 2.5e15},
         "history": {"plain": h_plain, "learned_in_loop": h_learn, "learned_in_loop_every5": h_every,
-                    "learned_output_every5": h_out, "learned_output_every5_discrepancy": h_disc},
+                    "learned_output_every5": h_out, "learned_output_every5_discrepancy": h_disc,
+                    "learned_dc_output_every5": h_dc5, "learned_dc_output_every10": h_dc10},
     }
     print(json.dumps(res))
 
@@ -400,6 +448,8 @@ def main():
     ap.add_argument("--epochs", type=int, default=3)
     ap.add_argument("--iters", type=int, default=60)
     ap.add_argument("--params", default="", help="--learned: trained parameters (.npy) instead of training")
+    ap.add_argument("--dc-steps", type=int, default=3,
+                    help="--learned: CG iterations of the learned step's data-consistency stage")
     ap.add_argument("--calib-copy-mb", type=int, default=0,
                     help="first run a plain device-to-device copy of this many MiB (profiler calibration)")
     args = ap.parse_args()

@@ -9,6 +9,11 @@
               accepted learned step the momentum restarts (t = 1, x_{k} := x_{k+1}), so
               the extrapolation does not amplify the network's correction
 
+The learned step may include a data-consistency stage (DataConsistent: a
+few CG iterations on ||Ax - b||^2 after the network), so that its output keeps
+the iterate's fit to the data while the network's correction removes
+artefacts; the safeguard is unchanged.
+
 With learned_every = N > 1 the learned step is tried every N-th iteration
 only (the others are plain FISTA-TV steps), so the network sees iterates close
 to the plain FISTA-TV ones it was trained on.  Output mode (feedback = False):
@@ -114,6 +119,25 @@ class ScaledCNN:
 
 
 EVAL_POPULATION = (512, 512, 64)   # ellipsoid population box (XN, YN, ZN) of the evaluation phantom
+
+
+class DataConsistent:
+    """Learned step with a data-consistency stage: x_c = x + net(x), then m
+    CG iterations on ||Ax - b||^2 started from x_c (the reconstruction's own
+    CG on the normal equations).  The network moves the estimate towards the
+    phantom class, which also undoes the part of the iterate's fit that
+    follows the noise; the CG steps restore the fit along the directions the
+    data constrain most, while the correction stays in the directions they
+    constrain little (the artefacts of sparse views).  The safeguard then
+    applies to the result as to any learned step."""
+
+    def __init__(self, net, rc, m=3):
+        self.net, self.rc, self.m = net, rc, int(m)
+
+    def __call__(self, x, out):
+        self.net(x, out)
+        if self.m > 0:
+            self.rc.cg(out, self.m)
 
 
 def noise_energy(b, I0):

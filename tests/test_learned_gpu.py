@@ -44,3 +44,21 @@ def test_trained_prox_gains_over_identity_and_safeguard_holds(cuda):
     _, plain, _ = LE.fista_tv_run(rc, L, g.vol_shape, 15, 0.3, 10, ph, None, cuda)
     print("PSNR after 15 iterations: plain %.2f dB, learned %.2f dB (%d learned steps accepted)"
           % (plain[-1]["psnr"], hist[-1]["psnr"], sum(h["learned"] for h in hist)))
+
+    # data-consistency stage: CG steps after the network lower ||Ax - b||^2 of
+    # the network's output, and the safeguard still holds with it
+    _, _, snaps = LE.fista_tv_run(rc, L, g.vol_shape, 8, 0.3, 10, ph, None, cuda, snaps=[8])
+    x8 = snaps[8]
+    raw = torch.empty_like(x8)
+    net(x8, raw)
+    dcs = LE.DataConsistent(net, rc, 3)
+    out = torch.empty_like(x8)
+    dcs(x8, out)
+    f_raw, f_dc = rc.data_fidelity(raw), rc.data_fidelity(out)
+    print(f"data consistency: ||Ax-b||^2 network output {f_raw:.4g}, after 3 CG steps {f_dc:.4g}")
+    assert f_dc < f_raw
+    _, hd, _ = LE.fista_tv_run(rc, L, g.vol_shape, 15, 0.3, 10, ph, None, cuda, learned=dcs, learned_every=3,
+                               feedback=False)
+    for a, b in zip(hd, plain):
+        if a["learned"]:
+            assert a["fidelity"] <= b["fidelity"], (a, b)     # output mode: against the plain iterate
