@@ -63,7 +63,7 @@ struct ctlab_plan {
     int sat_bits = 64;
     size_t ws_elem = 0;
     bool ws_shared = false; // d_cz / d_P belong to another plan (subset plans of a reconstruction)
-    int tile_nu = 16, tile_nm = 16;
+    int tile_nu = 8, tile_nm = 16;     // the autotuner's choice at 512x512x64, 984 views
 
     ctlab::DevGeom dev() const
     {

@@ -288,6 +288,10 @@ def fista_tv_run(rc, L, b_shape_vol, iters, tv_lambda, tv_iters, truth, zsl, dev
     taken = {}
     elapsed = 0.0
     for k in range(iters):
+        tried = learned is not None and ((k + 1) % learned_every == 0 or k + 1 == iters)
+        # in-loop mode compares with f(x_k) at the next tried iteration: keep it current
+        nxt = learned is not None and ((k + 2) % learned_every == 0 or k + 2 == iters)
+
         def step():
             nonlocal t, f_x
             rc.gradient(y, g)
@@ -296,21 +300,23 @@ def fista_tv_run(rc, L, b_shape_vol, iters, tv_lambda, tv_iters, truth, zsl, dev
             nonlocal out, f_out, cand_psnr, cand_fid
             accepted = False
             cand_psnr = cand_fid = None
-            tried = learned is not None and ((k + 1) % learned_every == 0 or k + 1 == iters)
             if not in_loop or not tried:
                 RC.tv_prox_into(z, x, tv_lambda / L, tv_iters, shape, work)
-                f_x = rc.data_fidelity(x)
-                out, f_out = x, f_x
+                f_x = None
+                out, f_out = x, None
                 if tried:
+                    f_x = rc.data_fidelity(x)
                     learned(x, xc)
                     f_c = rc.data_fidelity(xc)
+                    f_out = f_x
                     if f_c <= f_x or (noise_level is not None and f_c <= noise_level):
                         out, f_out = xc, f_c
                         accepted = True
                     else:
                         cand_fid = f_c
-                        if truth is not None:
-                            cand_psnr = psnr(xc, truth, zsl)
+                elif in_loop and nxt:
+                    f_x = rc.data_fidelity(x)
+                    f_out = f_x
             else:
                 RC.tv_prox_into(z, xt, tv_lambda / L, tv_iters, shape, work)
                 learned(xt, xc)
@@ -321,10 +327,8 @@ def fista_tv_run(rc, L, b_shape_vol, iters, tv_lambda, tv_iters, truth, zsl, dev
                     accepted = True
                 else:
                     cand_fid = f_c
-                    if truth is not None:
-                        cand_psnr = psnr(xc, truth, zsl)
                     x.copy_(xt)
-                    f_x = rc.data_fidelity(x)
+                    f_x = rc.data_fidelity(x) if nxt else None
                 out, f_out = x, f_x
             if accepted and in_loop:   # momentum restart
                 t = 1.0
@@ -337,6 +341,14 @@ def fista_tv_run(rc, L, b_shape_vol, iters, tv_lambda, tv_iters, truth, zsl, dev
             return accepted
         acc, dt = _timed(step)
         elapsed += dt
+        # monitoring outside the timed region: the output's fidelity where the
+        # algorithm did not need it, PSNRs, snapshots
+        if f_out is None:
+            f_out = rc.data_fidelity(out)
+        if out is x:
+            f_x = f_out
+        if cand_fid is not None and truth is not None:
+            cand_psnr = psnr(xc, truth, zsl)
         if snaps is not None and (k + 1) in snaps:
             taken[k + 1] = x.clone()
         hist.append({"iter": k + 1, "t": elapsed, "psnr": psnr(out, truth, zsl) if truth is not None else None,

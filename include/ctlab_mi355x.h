@@ -122,7 +122,7 @@ int ctlab_plan_set_sat_bits(ctlab_plan* plan, int bits);
 int ctlab_proj(ctlab_plan* plan, const float* d_vol, float* d_prj, void* stream);
 int ctlab_back(ctlab_plan* plan, const float* d_prj, float* d_vol, void* stream);
 /* tile shapes: detector columns per projector wavefront (nu) and voxel rows
- * per backprojector wavefront (nm), each 8, 12 or 16 (default 16 / 16) */
+ * per backprojector wavefront (nm), each 8, 12 or 16 (default 8 / 16) */
 int ctlab_plan_set_tiles(ctlab_plan* plan, int nu, int nm);
 int ctlab_plan_get_tiles(const ctlab_plan* plan, int* nu, int* nm);
 /* autotuner: times every tile variant of both operators on these device
