@@ -130,3 +130,28 @@ def test_driver_detector_is_the_arc_detector():
     for a, b in zip(got, ref):
         assert np.allclose(a, b, rtol=0, atol=2e-5 * max(1.0, float(np.abs(b).max())))
     assert np.array_equal(got[2], ref[2])
+
+
+def test_cxx_header_reference_names(tmp_path):
+    """A C++ program written against the reference's names (readData_host,
+    getVolZIdxPair, OS_SART, ...) compiles with include/ctlab_mi355x.hpp and
+    runs against the library."""
+    import os
+    import shutil
+    import subprocess
+    gxx = shutil.which("g++")
+    if gxx is None:
+        pytest.skip("no g++")
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    libdir = os.path.join(root, "ctlab_amd")
+    exe = tmp_path / "hpp_api"
+    subprocess.run([gxx, "-std=c++17", "-O1", "-I", os.path.join(root, "include"),
+                    os.path.join(root, "tests", "cxx", "hpp_api.cpp"), "-L", libdir, "-lctlab_mi355x",
+                    f"-Wl,-rpath,{libdir}", "-o", str(exe)], check=True)
+    out = subprocess.run([str(exe), str(tmp_path)], check=True, capture_output=True, text=True).stdout.split()
+    assert [float(v) for v in out[:5]] == [1.5, -2.25, 3.0, 0.1, 0.2]
+    s, e, ps, pe, threw = (int(v) for v in out[5:])
+    z = np.array([-6.0, -2.0, 2.0, 6.0], np.float32)
+    assert (s, e) == dd.getVolZIdxPair(z, 1, 3, 1.5, 2.0, 4, 7.5, 1.0, 16)
+    assert (ps, pe) == dd.getPrjIdxPair(z, 4, 8, 7.5, 1.0, 16, 1.5, 2.0, 4)
+    assert threw == 1
