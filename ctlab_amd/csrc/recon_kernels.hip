@@ -179,53 +179,65 @@ k_sum_partials(const double* __restrict__ part, int nb, int stride, int nsums, d
 }
 
 // ---- FGP TV prox on a (Z,X,Y) volume: p holds 3 components of n each ------
+// Launch: grid (ceil(XN*ZN / 256), YN), one thread per voxel of row y; the
+// in-row index j = x*ZN + z splits by a float reciprocal of ZN and one
+// correction step (exact for j < 2^22 * ZN: the estimate is off by at most 1),
+// instead of 64-bit divisions of the flat index.
+struct RowIdx { int x, z; long i; bool ok; };
+__device__ __forceinline__ RowIdx row_index(int XN, int ZN, float rcpZN)
+{
+    RowIdx r;
+    const int j = blockIdx.x * 256 + threadIdx.x;
+    r.ok = j < XN * ZN;
+    int q = (int)((float)j * rcpZN);
+    int z = j - q * ZN;
+    if (z < 0) { --q; z += ZN; } else if (z >= ZN) { ++q; z -= ZN; }
+    r.x = q;
+    r.z = z;
+    r.i = (long)blockIdx.y * XN * ZN + j;
+    return r;
+}
+
 // tmp = b - lam * L(r); L(r) = sum_d r_d[i] - r_d[i - e_d] (0 outside / at the far face)
-__global__ void k_tv_primal(const float* __restrict__ b, const float* __restrict__ r, float lam,
-                            int XN, int YN, int ZN, float* __restrict__ out)
+__global__ void __launch_bounds__(256) k_tv_primal(const float* __restrict__ b, const float* __restrict__ r, float lam,
+                                                   int XN, int YN, int ZN, float rcpZN, float* __restrict__ out)
 {
     const long n = (long)XN * YN * ZN;
+    const RowIdx q = row_index(XN, ZN, rcpZN);
+    if (!q.ok) return;
+    const long i = q.i;
     const float* rx = r;
     const float* ry = r + n;
     const float* rz = r + 2 * n;
-    GRID_STRIDE(i, n)
-    {
-        const int z = (int)(i % ZN);
-        const long xy = i / ZN;
-        const int x = (int)(xy % XN);
-        const int y = (int)(xy / XN);
-        float v = rx[i] + ry[i] + rz[i];
-        if (x > 0) v -= rx[i - ZN];
-        if (y > 0) v -= ry[i - (long)XN * ZN];
-        if (z > 0) v -= rz[i - 1];
-        out[i] = b[i] - lam * v;
-    }
+    float v = rx[i] + ry[i] + rz[i];
+    if (q.x > 0) v -= rx[i - ZN];
+    if (blockIdx.y > 0) v -= ry[i - (long)XN * ZN];
+    if (q.z > 0) v -= rz[i - 1];
+    out[i] = b[i] - lam * v;
 }
 
 // p_new = proj_{|.|<=1}(r + step * L^T(tmp)); r = p_new + mom*(p_new - p_old); p = p_new
-__global__ void k_tv_dual(const float* __restrict__ tmp, float* __restrict__ p, float* __restrict__ r,
-                          float step, float mom, int XN, int YN, int ZN)
+__global__ void __launch_bounds__(256) k_tv_dual(const float* __restrict__ tmp, float* __restrict__ p,
+                                                 float* __restrict__ r, float step, float mom, int XN, int YN, int ZN,
+                                                 float rcpZN)
 {
     const long n = (long)XN * YN * ZN;
-    GRID_STRIDE(i, n)
-    {
-        const int z = (int)(i % ZN);
-        const long xy = i / ZN;
-        const int x = (int)(xy % XN);
-        const int y = (int)(xy / XN);
-        const float t = tmp[i];
-        const float qx = x < XN - 1 ? t - tmp[i + ZN] : 0.0f;
-        const float qy = y < YN - 1 ? t - tmp[i + (long)XN * ZN] : 0.0f;
-        const float qz = z < ZN - 1 ? t - tmp[i + 1] : 0.0f;
-        const float a0 = r[i] + step * qx, a1 = r[n + i] + step * qy, a2 = r[2 * n + i] + step * qz;
-        const float nrm = sqrtf(a0 * a0 + a1 * a1 + a2 * a2);
-        const float s = nrm > 1.0f ? 1.0f / nrm : 1.0f;
-        const float p0 = a0 * s, p1 = a1 * s, p2 = a2 * s;
-        const float o0 = p[i], o1 = p[n + i], o2 = p[2 * n + i];
-        p[i] = p0; p[n + i] = p1; p[2 * n + i] = p2;
-        r[i] = p0 + mom * (p0 - o0);
-        r[n + i] = p1 + mom * (p1 - o1);
-        r[2 * n + i] = p2 + mom * (p2 - o2);
-    }
+    const RowIdx q = row_index(XN, ZN, rcpZN);
+    if (!q.ok) return;
+    const long i = q.i;
+    const float t = tmp[i];
+    const float qx = q.x < XN - 1 ? t - tmp[i + ZN] : 0.0f;
+    const float qy = (int)blockIdx.y < YN - 1 ? t - tmp[i + (long)XN * ZN] : 0.0f;
+    const float qz = q.z < ZN - 1 ? t - tmp[i + 1] : 0.0f;
+    const float a0 = r[i] + step * qx, a1 = r[n + i] + step * qy, a2 = r[2 * n + i] + step * qz;
+    const float nrm = sqrtf(a0 * a0 + a1 * a1 + a2 * a2);
+    const float s = nrm > 1.0f ? 1.0f / nrm : 1.0f;
+    const float p0 = a0 * s, p1 = a1 * s, p2 = a2 * s;
+    const float o0 = p[i], o1 = p[n + i], o2 = p[2 * n + i];
+    p[i] = p0; p[n + i] = p1; p[2 * n + i] = p2;
+    r[i] = p0 + mom * (p0 - o0);
+    r[n + i] = p1 + mom * (p1 - o1);
+    r[2 * n + i] = p2 + mom * (p2 - o2);
 }
 
 // ---------------------------------------------------------------------------
@@ -311,19 +323,22 @@ hipError_t rk_tv_prox(const float* b, float* x, float lam, int iters, int XN, in
     if (lam <= 0.0f) {
         return hipMemcpyAsync(x, b, n * sizeof(float), hipMemcpyDeviceToDevice, s);
     }
+    if ((long)XN * ZN > (1L << 22) || YN > 65535) return hipErrorInvalidValue;   // row index and grid limits
+    const dim3 grid((unsigned)(((long)XN * ZN + 255) / 256), (unsigned)YN);
+    const float rcp = 1.0f / (float)ZN;
     const int nd = (XN > 1) + (YN > 1) + (ZN > 1);
     const float step = 1.0f / (4.0f * (float)(nd > 0 ? nd : 1) * lam);
     hipLaunchKernelGGL(k_fill, dim3(ew_grid(3 * n)), dim3(kEwBlock), 0, s, p, 0.0f, 3 * n);
     hipLaunchKernelGGL(k_fill, dim3(ew_grid(3 * n)), dim3(kEwBlock), 0, s, r, 0.0f, 3 * n);
     double t = 1.0;
     for (int it = 0; it < iters; ++it) {
-        hipLaunchKernelGGL(k_tv_primal, dim3(ew_grid(n)), dim3(kEwBlock), 0, s, b, r, lam, XN, YN, ZN, tmp);
+        hipLaunchKernelGGL(k_tv_primal, grid, dim3(256), 0, s, b, r, lam, XN, YN, ZN, rcp, tmp);
         const double tn = 0.5 * (1.0 + __builtin_sqrt(1.0 + 4.0 * t * t));
         const float mom = (float)((t - 1.0) / tn);
-        hipLaunchKernelGGL(k_tv_dual, dim3(ew_grid(n)), dim3(kEwBlock), 0, s, tmp, p, r, step, mom, XN, YN, ZN);
+        hipLaunchKernelGGL(k_tv_dual, grid, dim3(256), 0, s, tmp, p, r, step, mom, XN, YN, ZN, rcp);
         t = tn;
     }
-    hipLaunchKernelGGL(k_tv_primal, dim3(ew_grid(n)), dim3(kEwBlock), 0, s, b, p, lam, XN, YN, ZN, x);
+    hipLaunchKernelGGL(k_tv_primal, grid, dim3(256), 0, s, b, p, lam, XN, YN, ZN, rcp, x);
     return hipGetLastError();
 }
 
