@@ -68,6 +68,18 @@ def test_back_parity_cone(cuda, arc):
     assert e32 <= TOL32
 
 
+def test_dense_view_set_parity(cuda):
+    """cfg2's 888-column detector with 448 views: many more projector blocks
+    (111 tiles x 112 view groups) and start-table entries than the small cases."""
+    g = _cone(XN=64, YN=56, ZN=16, PN=448, DNU=888, DNV=16, col_size=0.3)
+    vol = _phantom(g, 51)
+    e_p = rel_l2(_run(g, vol=vol), O.proj(g, vol))
+    prj = np.random.default_rng(52).random(g.nprj()).astype(np.float32)
+    e_b = rel_l2(_run(g, prj=prj), O.back(g, prj))
+    print(f"888 x 448 views: proj {e_p:.2e}, back {e_b:.2e}")
+    assert e_p <= TOL64 and e_b <= TOL64
+
+
 @pytest.mark.parametrize("arc", [True, False])
 def test_helical_parity(cuda, arc):
     """Helical scan (hzPos != 0: 2 turns, 15 mm per turn through a 36 mm
