@@ -21,7 +21,7 @@ __global__ void k_wgrad_first(const __bf16*, const float*, float*, int, int, int
 __global__ void k_wgrad_last(const float*, const __bf16*, float*, int, int, int, long);
 __global__ void k_bias_grad_partial(const __bf16*, long, long, float*);
 __global__ void k_sum_partial(const float*, long, long, float*);
-__global__ void k_reduce_chunks(const float*, int, int, float*);
+__global__ void k_reduce_chunks(const float*, int, int, int, float*);
 __global__ void k_flip_transpose(const __bf16*, __bf16*);
 __global__ void k_adam(float*, const float*, float*, float*, long, float, float, float, float, float, float);
 __global__ void k_to_bf16(const float*, __bf16*, float*, long);
@@ -40,6 +40,7 @@ constexpr long kLenB[5] = {CH, CH, CH, CH, 1};
 constexpr long kNParams = 27 * CH + CH + 3 * (kW32 + CH) + 27 * CH + 1;
 constexpr long kWgradChunk = 8192;     // voxels per wgrad partial (multiple of 32)
 constexpr long kVecChunk = 512;        // voxels per VALU wgrad partial
+constexpr long kRedTmp = 2048 * 64;    // reduce_chunks: groups x outputs <= (2048 / ceil(n/64)) * n
 }  // namespace
 
 struct ctlab_cnn {
@@ -57,6 +58,7 @@ struct ctlab_cnn {
     float* out = nullptr;
     float* g5 = nullptr;
     float* part = nullptr;
+    float* rtmp = nullptr;              // reduce_chunks group partials (kRedTmp floats)
     double* lpart = nullptr;
     long nchunks = 0, nvchunks = 0;
     long adam_t = 0;
@@ -73,7 +75,7 @@ void cnn_free(ctlab_cnn* N)
     DeviceGuard dg(N->device);
     void* ptrs[] = {N->p, N->g, N->m1, N->m2, N->wr, N->wb, N->wt, N->a_cl[0], N->a_cl[1], N->a_cl[2], N->a_cl[3],
                     N->a_cm[0], N->a_cm[1], N->a_cm[2], N->a_cm[3], N->g_cl[0], N->g_cl[1], N->g_cm[0],
-                    N->g_cm[1], N->out, N->g5, N->part, N->lpart};
+                    N->g_cm[1], N->out, N->g5, N->part, N->rtmp, N->lpart};
     for (void* q : ptrs)
         if (q) hipFree(q);
     delete N;
@@ -117,6 +119,23 @@ int forward(ctlab_cnn* N, const float* x, float* out, const float* target, hipSt
     return chk(hipGetLastError(), "cnn forward");
 }
 
+// out[i] = sum over nchunks partial rows of part (row c at part + c*n), in a
+// fixed order: chunk groups spread over ~2048 blocks write partial sums to
+// N->rtmp, which a second pass adds up.
+void reduce_chunks(ctlab_cnn* N, const float* part, int nchunks, int n, float* out, hipStream_t s)
+{
+    const int gx = (n + 63) / 64;
+    int G = std::max(1, std::min(2048 / gx, nchunks / 16));
+    const int per = (nchunks + G - 1) / G;
+    G = (nchunks + per - 1) / per;
+    if (G == 1) {
+        hipLaunchKernelGGL(k_reduce_chunks, dim3(gx, 1), dim3(256), 0, s, part, nchunks, nchunks, n, out);
+    } else {
+        hipLaunchKernelGGL(k_reduce_chunks, dim3(gx, G), dim3(256), 0, s, part, nchunks, per, n, N->rtmp);
+        hipLaunchKernelGGL(k_reduce_chunks, dim3(gx, 1), dim3(256), 0, s, (const float*)N->rtmp, G, G, n, out);
+    }
+}
+
 int backward(ctlab_cnn* N, const float* x, hipStream_t s)
 {
     const int XN = N->XN, YN = N->YN, ZN = N->ZN;
@@ -125,11 +144,9 @@ int backward(ctlab_cnn* N, const float* x, hipStream_t s)
     // layer 5
     hipLaunchKernelGGL(k_wgrad_last, dim3((unsigned)N->nvchunks), dim3(256), 0, s, N->g5, N->a_cl[3], N->part, XN, YN,
                        ZN, kVecChunk);
-    hipLaunchKernelGGL(k_reduce_chunks, dim3(blocks(27 * CH, 256)), dim3(256), 0, s, N->part, (int)N->nvchunks,
-                       27 * CH, G + kOffW[4]);
+    reduce_chunks(N, N->part, (int)N->nvchunks, 27 * CH, G + kOffW[4], s);
     hipLaunchKernelGGL(k_sum_partial, dim3((unsigned)N->nvchunks), dim3(256), 0, s, N->g5, n, kVecChunk, N->part);
-    hipLaunchKernelGGL(k_reduce_chunks, dim3(1), dim3(256), 0, s, N->part, (int)N->nvchunks, 1,
-                       G + kOffW[4] + kLenW[4]);
+    reduce_chunks(N, N->part, (int)N->nvchunks, 1, G + kOffW[4] + kLenW[4], s);
     hipLaunchKernelGGL(k_dgrad_last, dim3(blocks(n, 256)), dim3(256), 0, s, N->g5, N->wr + 27 * CH, N->a_cl[3],
                        N->g_cl[0], N->g_cm[0], XN, YN, ZN);
     int cur = 0;   // g_*[cur] holds dL/dz of layer l+1 (l = 3, 2, 1 below: layers 4, 3, 2)
@@ -137,12 +154,10 @@ int backward(ctlab_cnn* N, const float* x, hipStream_t s)
         // weight / bias gradients of layer l+1 (index l): input a_{l} = a_cl[l-1]
         hipLaunchKernelGGL(k_wgrad32, dim3((unsigned)N->nchunks, 9), dim3(64), 0, s, N->g_cm[cur], N->a_cm[l - 1],
                            N->part, XN, YN, ZN, kWgradChunk);
-        hipLaunchKernelGGL(k_reduce_chunks, dim3(blocks(kW32, 256)), dim3(256), 0, s, N->part, (int)N->nchunks,
-                           (int)kW32, G + kOffW[l]);
+        reduce_chunks(N, N->part, (int)N->nchunks, (int)kW32, G + kOffW[l], s);
         hipLaunchKernelGGL(k_bias_grad_partial, dim3((unsigned)N->nvchunks), dim3(256), 0, s, N->g_cm[cur], n,
                            kVecChunk, N->part);
-        hipLaunchKernelGGL(k_reduce_chunks, dim3(1), dim3(256), 0, s, N->part, (int)N->nvchunks, CH,
-                           G + kOffW[l] + kLenW[l]);
+        reduce_chunks(N, N->part, (int)N->nvchunks, CH, G + kOffW[l] + kLenW[l], s);
         // input gradient through layer l+1, masked by the ReLU of layer l
         hipLaunchKernelGGL(k_conv32, dim3(blocks(n, 256)), dim3(256), 0, s, N->g_cl[cur], N->wt + (l - 1) * kW32,
                            (const float*)nullptr, N->a_cl[l - 1], N->g_cl[cur ^ 1], N->g_cm[cur ^ 1], XN, YN, ZN, 0);
@@ -151,12 +166,10 @@ int backward(ctlab_cnn* N, const float* x, hipStream_t s)
     // layer 1
     hipLaunchKernelGGL(k_wgrad_first, dim3((unsigned)N->nvchunks), dim3(256), 0, s, N->g_cl[cur], x, N->part, XN, YN,
                        ZN, kVecChunk);
-    hipLaunchKernelGGL(k_reduce_chunks, dim3(blocks(27 * CH, 256)), dim3(256), 0, s, N->part, (int)N->nvchunks,
-                       27 * CH, G + kOffW[0]);
+    reduce_chunks(N, N->part, (int)N->nvchunks, 27 * CH, G + kOffW[0], s);
     hipLaunchKernelGGL(k_bias_grad_partial, dim3((unsigned)N->nvchunks), dim3(256), 0, s, N->g_cm[cur], n, kVecChunk,
                        N->part);
-    hipLaunchKernelGGL(k_reduce_chunks, dim3(1), dim3(256), 0, s, N->part, (int)N->nvchunks, CH,
-                       G + kOffW[0] + kLenW[0]);
+    reduce_chunks(N, N->part, (int)N->nvchunks, CH, G + kOffW[0] + kLenW[0], s);
     return chk(hipGetLastError(), "cnn backward");
 }
 }  // namespace
@@ -201,6 +214,7 @@ ctlab_cnn* ctlab_cnn_create(int XN, int YN, int ZN, int training, int gpunum)
         const size_t part = std::max((size_t)N->nchunks * kW32, (size_t)N->nvchunks * 27 * CH) * sizeof(float);
         ok = ok && !chk(hipMalloc(&N->g, P), "hipMalloc") && !chk(hipMalloc(&N->m1, P), "hipMalloc") &&
              !chk(hipMalloc(&N->m2, P), "hipMalloc") && !chk(hipMalloc(&N->part, part), "hipMalloc") &&
+             !chk(hipMalloc(&N->rtmp, kRedTmp * sizeof(float)), "hipMalloc") &&
              !chk(hipMemset(N->m1, 0, P), "memset") && !chk(hipMemset(N->m2, 0, P), "memset");
     }
     if (!ok) { cnn_free(N); return nullptr; }

@@ -451,14 +451,26 @@ k_sum_partial(const float* __restrict__ x, long n, long chunk, float* __restrict
     if (threadIdx.x == 0) part[blockIdx.x] = sh[0] + sh[1] + sh[2] + sh[3];
 }
 
-// out[i] = sum_c part[c][i]  (fixed order over chunks)
-__global__ void k_reduce_chunks(const float* __restrict__ part, int nchunks, int n, float* __restrict__ out)
+// out[y*n + i] = sum of part[c*n + i] over the chunks c of group y
+// ([y*per, min((y+1)*per, nchunks))).  Block = 64 outputs (lanes) x 4
+// wavefronts, each summing one quarter of the group's chunks in order; the
+// quarters are added in a fixed order, so the result does not depend on
+// scheduling.
+__global__ void __launch_bounds__(256)
+k_reduce_chunks(const float* __restrict__ part, int nchunks, int per, int n, float* __restrict__ out)
 {
-    const int i = blockIdx.x * 256 + threadIdx.x;
-    if (i >= n) return;
+    __shared__ float s4[4][64];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int i = blockIdx.x * 64 + lane;
+    const int c0 = blockIdx.y * per, c1 = min(c0 + per, nchunks);
+    const int q = (c1 - c0 + 3) / 4;
+    const int a = c0 + w * q, b = min(a + q, c1);
     float s = 0.0f;
-    for (int c = 0; c < nchunks; ++c) s += part[(long)c * n + i];
-    out[i] = s;
+    if (i < n)
+        for (int c = a; c < b; ++c) s += part[(long)c * n + i];
+    s4[w][lane] = s;
+    __syncthreads();
+    if (w == 0 && i < n) out[(long)blockIdx.y * n + i] = (s4[0][lane] + s4[1][lane]) + (s4[2][lane] + s4[3][lane]);
 }
 
 // flipped, transposed weights for the input gradient: wt[t][ci][co] = w[26-t][co][ci]
