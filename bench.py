@@ -60,8 +60,7 @@ def cpu_baseline(g, vol, prj_gpu, back_fn, nviews):
     work = 2.0 * g.nvox() * nviews
     p_gpu = prj_gpu[:nviews].reshape(-1).astype(np.float64)
     b_gpu = back_fn(sub, prj_gpu[:nviews]).reshape(-1).astype(np.float64)
-    e_p = float(np.linalg.norm(p_gpu - p_ref) / np.linalg.norm(p_ref))
-    e_b = float(np.linalg.norm(b_gpu - b_ref) / np.linalg.norm(b_ref))
+    par = errors(p_gpu, p_ref, b_gpu, b_ref)
     threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
     return {
         "value": work / (t2 - t0),
@@ -70,7 +69,16 @@ def cpu_baseline(g, vol, prj_gpu, back_fn, nviews):
         "kind": "port",
         "sample": f"FP64 oracle (OpenMP) proj + back of cfg2 geometry restricted to views 0..{nviews - 1}, "
                   f"full 512x512x64 volume and 888x64 detector; proj {t1 - t0:.2f} s, back {t2 - t1:.2f} s",
-    }, {"proj_rel_l2_vs_fp64": e_p, "back_rel_l2_vs_fp64": e_b, "views_checked": nviews}
+    }, dict(par, views_checked=nviews), (sub, p_ref, b_ref)
+
+
+def errors(p_gpu, p_ref, b_gpu, b_ref):
+    """relative L2 and max-abs error of a GPU projection / backprojection against FP64 references"""
+    rel = lambda a, r: float(np.linalg.norm(a - r) / np.linalg.norm(r))
+    return {"proj_rel_l2_vs_fp64": rel(p_gpu, p_ref), "back_rel_l2_vs_fp64": rel(b_gpu, b_ref),
+            "proj_max_abs_vs_fp64": float(np.max(np.abs(p_gpu - p_ref))),
+            "back_max_abs_vs_fp64": float(np.max(np.abs(b_gpu - b_ref))),
+            "proj_max_abs_ref": float(np.max(np.abs(p_ref))), "back_max_abs_ref": float(np.max(np.abs(b_ref)))}
 
 
 def target_report(h_plain, runs, target):
@@ -557,21 +565,56 @@ def main():
                      "traffic": ((traffic or {}).get("by_tiles", {}).get("%d,%d" % plan.tiles(), {})
                                  .get(dominant + "_bytes_per_call")),
                      "kernel": "dd_proj_walk (+zprefix)" if dominant == "proj"
-                               else "dd_back_walk x2 (+prjprefix)",
+                               else "dd_back_walk (+prjprefix, back_add)",
                      "pair_frac": (2 * bytes_op / t_pair) / HBM_PEAK},
         "ms_proj": float(np.mean(tp)) * 1e3,
         "ms_back": float(np.mean(tb)) * 1e3,
         "wall_s": wall,
     }
+    # adjoint identity at full size: x, y ~ U[0, 1) (torch generators seeded 3 and 4), dot products in FP64
+    gx = torch.Generator(device=dev)
+    gx.manual_seed(3)
+    xa = torch.rand(g.vol_shape, generator=gx, device=dev)
+    gx.manual_seed(4)
+    ya = torch.rand(g.prj_shape, generator=gx, device=dev)
+    lhs = float((plan.proj(xa).double() * ya.double()).sum())
+    rhs = float((xa.double() * plan.back(ya).double()).sum())
+    out["adjoint"] = {"ax_y": lhs, "x_aty": rhs, "rel_mismatch": abs(lhs - rhs) / abs(lhs)}
+    del xa, ya
+    # FP32 integral-image variant at the same tiles
+    p32 = dd.Plan(g, sat_bits=32)
+    p32.set_tiles(*plan.tiles())
+    prj32 = torch.empty_like(prj)
+    bak32 = torch.empty_like(bak)
+    for _ in range(max(1, args.warmup)):
+        p32.proj(vol, out=prj32)
+        p32.back(prj32, out=bak32)
+    e32 = [torch.cuda.Event(enable_timing=True) for _ in range(3)]
+    t32p, t32b = [], []
+    for _ in range(3):
+        e32[0].record(stream)
+        p32.proj(vol, out=prj32)
+        e32[1].record(stream)
+        p32.back(prj32, out=bak32)
+        e32[2].record(stream)
+        torch.cuda.synchronize()
+        t32p.append(e32[0].elapsed_time(e32[1]))
+        t32b.append(e32[1].elapsed_time(e32[2]))
+    out["fp32_sat"] = {"ms_proj": float(np.mean(t32p)), "ms_back": float(np.mean(t32b)),
+                       "voxel_views_per_s": 2.0 * vv / (1e-3 * (np.mean(t32p) + np.mean(t32b)))}
     if not args.no_cpu:
-        def back_fn(sub, p_np):
-            pl = dd.Plan(sub, sat_bits=args.sat_bits)
+        def back_fn(sub, p_np, bits=args.sat_bits):
+            pl = dd.Plan(sub, sat_bits=bits)
             r = pl.back(torch.from_numpy(np.ascontiguousarray(p_np)).to(dev))
             torch.cuda.synchronize()
             return r.cpu().numpy()
-        cb, par = cpu_baseline(g, vol_np, prj.cpu().numpy(), back_fn, args.cpu_views)
+        cb, par, (sub, p_ref, b_ref) = cpu_baseline(g, vol_np, prj.cpu().numpy(), back_fn, args.cpu_views)
         out["cpu_baseline"] = cb
         out["parity"] = par
+        n = args.cpu_views
+        out["fp32_sat"]["parity"] = errors(prj32[:n].cpu().numpy().reshape(-1).astype(np.float64), p_ref,
+                                           back_fn(sub, prj.cpu().numpy()[:n], 32).reshape(-1).astype(np.float64),
+                                           b_ref)
     print(json.dumps(out))
 
 
