@@ -366,32 +366,60 @@ k_wgrad32(const __bf16* __restrict__ g_cm, const __bf16* __restrict__ a_cm, floa
     }
 }
 
+// Thread (t, c) pair of a first / last layer weight gradient: tap offset and
+// displacement, fixed over the voxel loop (threads past 27*32 stay idle).
+struct TapPair {
+    int off, dy, dx, dz, c;
+    bool live;
+};
+__device__ __forceinline__ TapPair tap_pair(int idx, int XN, int ZN)
+{
+    TapPair q;
+    q.live = idx < 27 * CH;
+    const int t = q.live ? idx / CH : 0;
+    q.c = idx % CH;
+    q.off = tap_off(t, XN, ZN);
+    q.dy = t / 9 - 1;
+    q.dx = (t / 3) % 3 - 1;
+    q.dz = t % 3 - 1;
+    return q;
+}
+__device__ __forceinline__ bool tap_in(const TapPair& q, int y, int x, int z, int YN, int XN, int ZN)
+{
+    return q.live && (unsigned)(y + q.dy) < (unsigned)YN && (unsigned)(x + q.dx) < (unsigned)XN &&
+           (unsigned)(z + q.dz) < (unsigned)ZN;
+}
+
 // weight gradient of the first layer: dW1[t][co] = sum_v g1[v][co] * x[v + off(t)]
 // one block per chunk; thread (t, co) pairs; partials part[chunk][27*32].  The
 // gradient is read channel-last (g_cl[v][co]): the 64 lanes of a wavefront,
-// two taps x 32 channels, then read one 64-byte line per voxel.
+// two taps x 32 channels, then read one 64-byte line per voxel.  The voxel
+// coordinates step along with v (z fastest), without divisions in the loop.
 __global__ void __launch_bounds__(256)
 k_wgrad_first(const __bf16* __restrict__ g_cl, const float* __restrict__ x, float* __restrict__ part,
               int XN, int YN, int ZN, long chunk)
 {
     const long nvox = (long)XN * YN * ZN;
     const long c0 = blockIdx.x * chunk, c1 = min(c0 + chunk, nvox);
+    TapPair q[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) q[k] = tap_pair(threadIdx.x + 256 * k, XN, ZN);
     float acc[4] = {0.f, 0.f, 0.f, 0.f};
+    int z = (int)(c0 % ZN);
+    const long xy = c0 / ZN;
+    int xx = (int)(xy % XN), y = (int)(xy / XN);
     for (long v = c0; v < c1; ++v) {
-        const int z = (int)(v % ZN);
-        const long xy = v / ZN;
-        const int xx = (int)(xy % XN), y = (int)(xy / XN);
-        for (int k = 0; k < 4; ++k) {
-            const int idx = threadIdx.x + 256 * k;   // t*32 + co
-            if (idx >= 27 * CH) break;
-            const int t = idx / CH, co = idx % CH;
-            if (tap_ok(t, y, xx, z, YN, XN, ZN)) acc[k] += bf(g_cl[v * CH + co]) * x[v + tap_off(t, XN, ZN)];
+#pragma unroll
+        for (int k = 0; k < 4; ++k)
+            if (tap_in(q[k], y, xx, z, YN, XN, ZN)) acc[k] += bf(g_cl[v * CH + q[k].c]) * x[v + q[k].off];
+        if (++z == ZN) {
+            z = 0;
+            if (++xx == XN) { xx = 0; ++y; }
         }
     }
-    for (int k = 0; k < 4; ++k) {
-        const int idx = threadIdx.x + 256 * k;
-        if (idx < 27 * CH) part[(long)blockIdx.x * 27 * CH + idx] = acc[k];
-    }
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+        if (q[k].live) part[(long)blockIdx.x * 27 * CH + threadIdx.x + 256 * k] = acc[k];
 }
 
 // weight gradient of the last layer: dW5[t][ci] = sum_v g5[v] * a4[v + off(t)][ci]
@@ -401,23 +429,26 @@ k_wgrad_last(const float* __restrict__ g5, const __bf16* __restrict__ a_cl, floa
 {
     const long nvox = (long)XN * YN * ZN;
     const long c0 = blockIdx.x * chunk, c1 = min(c0 + chunk, nvox);
+    TapPair q[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) q[k] = tap_pair(threadIdx.x + 256 * k, XN, ZN);
     float acc[4] = {0.f, 0.f, 0.f, 0.f};
+    int z = (int)(c0 % ZN);
+    const long xy = c0 / ZN;
+    int xx = (int)(xy % XN), y = (int)(xy / XN);
     for (long v = c0; v < c1; ++v) {
-        const int z = (int)(v % ZN);
-        const long xy = v / ZN;
-        const int xx = (int)(xy % XN), y = (int)(xy / XN);
         const float gv = g5[v];
-        for (int k = 0; k < 4; ++k) {
-            const int idx = threadIdx.x + 256 * k;   // t*32 + ci
-            if (idx >= 27 * CH) break;
-            const int t = idx / CH, ci = idx % CH;
-            if (tap_ok(t, y, xx, z, YN, XN, ZN)) acc[k] += gv * bf(a_cl[(v + tap_off(t, XN, ZN)) * CH + ci]);
+#pragma unroll
+        for (int k = 0; k < 4; ++k)
+            if (tap_in(q[k], y, xx, z, YN, XN, ZN)) acc[k] += gv * bf(a_cl[(v + q[k].off) * CH + q[k].c]);
+        if (++z == ZN) {
+            z = 0;
+            if (++xx == XN) { xx = 0; ++y; }
         }
     }
-    for (int k = 0; k < 4; ++k) {
-        const int idx = threadIdx.x + 256 * k;
-        if (idx < 27 * CH) part[(long)blockIdx.x * 27 * CH + idx] = acc[k];
-    }
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+        if (q[k].live) part[(long)blockIdx.x * 27 * CH + threadIdx.x + 256 * k] = acc[k];
 }
 
 // bias-gradient partials: part[j][c] = sum over chunk j of g_cm[c][v] (block j;
