@@ -143,7 +143,8 @@ def learned_bench(args):
                                              kmin=5, kmax=args.iters, log=log)
         loss_id = LE.identity_loss(xs, ts)
         log(f"identity-map loss {loss_id:.4e}")
-        params, losses, t_train, steps = LE.train(xs, ts, tshape, args.epochs, lr=1e-3, log=log)
+        params, losses, t_train, steps = LE.train(xs, ts, tshape, args.epochs, lr=1e-3, log=log,
+                                                  schedule=args.lr_schedule)
         del xs, ts
         np.save(os.path.join(ROOT, "gpurun_out", "learned_prox_params.npy"), params) if os.path.isdir(
             os.path.join(ROOT, "gpurun_out")) else None
@@ -179,6 +180,12 @@ def learned_bench(args):
                                   learned_every=5, feedback=False)
     _, h_dc10, _ = LE.fista_tv_run(re_, Le, ge.vol_shape, args.iters, lam, 10, ph, ze, dev, learned=dc,
                                    learned_every=10, feedback=False)
+    # adaptive stage: CG until the candidate's ||Ax-b||^2 is at the plain iterate's (equal data residual)
+    dca = LE.DataConsistent(net, re_, to_fidelity=True, m_max=args.dc_max)
+    _, h_dca5, _ = LE.fista_tv_run(re_, Le, ge.vol_shape, args.iters, lam, 10, ph, ze, dev, learned=dca,
+                                   learned_every=5, feedback=False)
+    _, h_dca10, _ = LE.fista_tv_run(re_, Le, ge.vol_shape, args.iters, lam, 10, ph, ze, dev, learned=dca,
+                                    learned_every=10, feedback=False)
     target = 40.0
     # report only (the weight above is the one chosen without the evaluation phantom): plain
     # FISTA-TV on the evaluation phantom itself for nearby weights and longer runs
@@ -220,6 +227,7 @@ def learned_bench(args):
         "metric": METRIC, "mode": "cfg5 learned safeguarded FISTA vs plain FISTA-TV",
         "train": {"pairs": (args.samples // per) * per, "phantoms": args.samples // per, "snapshots_per_phantom": per,
                   "shape": list(tshape), "views": 120, "I0": 1e4, "epochs": args.epochs,
+                  "lr_schedule": args.lr_schedule,
                   "steps": steps, "gen_s": t_gen, "train_s": t_train, "loss_per_epoch": losses,
                   "loss_identity_map": loss_id, "params_loaded": bool(args.params)},
         "validation_256": {"plain_best_psnr": max(h["psnr"] for h in
@@ -242,16 +250,21 @@ def learned_bench(args):
         "learned_output_every5_discrepancy": summary(h_disc),
         "learned_dc_output_every5": summary(h_dc5),
         "learned_dc_output_every10": summary(h_dc10),
-        "dc_steps": args.dc_steps,
+        "learned_dc_adaptive_output_every5": summary(h_dca5),
+        "learned_dc_adaptive_output_every10": summary(h_dca10),
+        "dc_steps": args.dc_steps, "dc_max": args.dc_max,
         "target_40dB": target_report(h_plain, {"learned_dc_output_every10": h_dc10,
-                                               "learned_dc_output_every5": h_dc5}, target),
+                                               "learned_dc_output_every5": h_dc5,
+                                               "learned_dc_adaptive_output_every10": h_dca10,
+                                               "learned_dc_adaptive_output_every5": h_dca5}, target),
         "plain_eval_phantom_sweep": {"iters": args.plain_sweep_iters, "by_lambda": plain_sweep},
         "ref_psnr_db": ref_db,
         "cnn_forward": {"shape": [512, 512, 64], "s": t_fwd, "tflops": rate / 1e12,
                         "mfma_util_vs_bf16_dense_peak": rate / 2.5e15},
         "history": {"plain": h_plain, "learned_in_loop": h_learn, "learned_in_loop_every5": h_every,
                     "learned_output_every5": h_out, "learned_output_every5_discrepancy": h_disc,
-                    "learned_dc_output_every5": h_dc5, "learned_dc_output_every10": h_dc10},
+                    "learned_dc_output_every5": h_dc5, "learned_dc_output_every10": h_dc10,
+                    "learned_dc_adaptive_output_every5": h_dca5, "learned_dc_adaptive_output_every10": h_dca10},
     }
     print(json.dumps(res))
 
@@ -465,13 +478,17 @@ def main():
     ap.add_argument("--problems", type=int, default=32)
     ap.add_argument("--subsets", type=int, default=10)
     ap.add_argument("--samples", type=int, default=2000)
-    ap.add_argument("--epochs", type=int, default=3)
+    ap.add_argument("--epochs", type=int, default=8)
+    ap.add_argument("--lr-schedule", default="cosine", choices=("cosine", "step"),
+                    help="--learned: Adam step sizes (cosine decay to 0.05x, or the last epoch at 0.3x)")
     ap.add_argument("--iters", type=int, default=60)
     ap.add_argument("--params", default="", help="--learned: trained parameters (.npy) instead of training")
     ap.add_argument("--plain-sweep-iters", type=int, default=300,
                     help="--learned: iterations of the reported plain FISTA-TV weight sweep on the evaluation phantom")
     ap.add_argument("--dc-steps", type=int, default=3,
                     help="--learned: CG iterations of the learned step's data-consistency stage")
+    ap.add_argument("--dc-max", type=int, default=10,
+                    help="--learned: most CG iterations of the adaptive stage (CG until the plain iterate's residual)")
     ap.add_argument("--calib-copy-mb", type=int, default=0,
                     help="first run a plain device-to-device copy of this many MiB (profiler calibration)")
     args = ap.parse_args()

@@ -18,6 +18,7 @@ def declare(L):
         "ctlab_os_sart": ([_vp, _fp, i, f, i, i, _vp], i),
         "ctlab_fista_tv": ([_vp, _fp, i, f, i, f, i, _dp, _vp], i),
         "ctlab_cg": ([_vp, _fp, i, i, _dp, _vp], i),
+        "ctlab_cg_to_fidelity": ([_vp, _fp, i, d, _dp, C.POINTER(i), _vp], i),
         "ctlab_recon_data_fidelity": ([_vp, _fp, _vp], d),
         "ctlab_recon_gradient": ([_vp, _fp, _fp, _vp], d),
         "ctlab_recon_lipschitz": ([_vp, i, _vp], d),

@@ -129,7 +129,7 @@ namespace ctlab {
 // CG on the normal equations (body of ctlab_cg); after_iter(k, d_x), when
 // set, runs after iteration k and stops the run by returning non-zero
 int cg_run(ctlab_recon* R, float* d_x, int iters, int cgm, double* h_resid, hipStream_t s,
-           const std::function<int(int, const float*)>& after_iter)
+           const std::function<int(int, const float*)>& after_iter, CgStop* stop)
 {
     if (!R || !d_x || iters < 0 || cgm < 0 || cgm > 4 || !R->have_data) {
         set_error(CTLAB_ERR_ARG, "need recon with data, volume, iters >= 0 and 0 <= cgm <= 4");
@@ -143,16 +143,26 @@ int cg_run(ctlab_recon* R, float* d_x, int iters, int cgm, double* h_resid, hipS
     float* d = R->d_vol4;
     float* w = R->d_vol;
     // r = A^T (b - A x) = -gradient
-    if (std::isnan(ctlab_recon_gradient(R, d_x, r, stream))) return ctlab_last_error();
+    double f = ctlab_recon_gradient(R, d_x, r, stream);
+    if (std::isnan(f)) return ctlab_last_error();
+    if (stop) stop->done = 0;
+    const int steps = (stop && f <= stop->f_target) ? 0 : iters;   // already there: no step
     if (chk(rk_axpby(0.0f, r, -1.0f, r, n, s), "negate")) return ctlab_last_error();
     hipMemcpyAsync(d, r, n * sizeof(float), hipMemcpyDeviceToDevice, s);
-    for (int it = 0; it < iters; ++it) {
+    for (int it = 0; it < steps; ++it) {
         if (plan_proj(R->full, d, R->d_prj, s)) return ctlab_last_error();
         const double qq = dot(R, R->d_prj, nullptr, R->nprj, s);
         const double rd = dot(R, r, d, n, s);
         if (!(qq > 0)) break;
         const float alpha = (float)(rd / qq);
         if (chk(rk_axpby(alpha, d, 1.0f, d_x, n, s), "update1")) return ctlab_last_error();   // X + alpha D
+        if (stop) {
+            // ||A(x + alpha d) - b||^2 = f - 2 alpha <A^T(b - Ax), d> + alpha^2 ||Ad||^2
+            f += (double)alpha * ((double)alpha * qq - 2.0 * rd);
+            if (stop->h_fid) stop->h_fid[it] = f;
+            stop->done = it + 1;
+            if (f <= stop->f_target) break;
+        }
         if (plan_back(R->full, R->d_prj, w, s)) return ctlab_last_error();                     // tem2 = A^T A d
         hipMemcpyAsync(rn, r, n * sizeof(float), hipMemcpyDeviceToDevice, s);
         if (chk(rk_axpby(-alpha, w, 1.0f, rn, n, s), "update2")) return ctlab_last_error();   // R - alpha tem2
@@ -434,6 +444,21 @@ int ctlab_cg(ctlab_recon* R, float* d_x, int iters, int cgm, double* h_resid, vo
 {
     clear_error();
     return ctlab::cg_run(R, d_x, iters, cgm, h_resid, (hipStream_t)stream, nullptr);
+}
+
+// Fletcher-Reeves CG from d_x until ||Ax - b||^2 <= f_target (at most max_iters steps).
+int ctlab_cg_to_fidelity(ctlab_recon* R, float* d_x, int max_iters, double f_target, double* h_fid,
+                         int* iters_done, void* stream)
+{
+    clear_error();
+    if (!iters_done || std::isnan(f_target)) {
+        set_error(CTLAB_ERR_ARG, "need iters_done and a target that is not NaN");
+        return CTLAB_ERR_ARG;
+    }
+    ctlab::CgStop stop{f_target, h_fid, 0};
+    const int rc = ctlab::cg_run(R, d_x, max_iters, 0, nullptr, (hipStream_t)stream, nullptr, &stop);
+    *iters_done = stop.done;
+    return rc;
 }
 
 // ---- elementwise entry points (device pointers, stream-ordered) ---------

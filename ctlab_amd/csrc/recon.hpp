@@ -39,7 +39,16 @@ struct ctlab_recon {
 
 namespace ctlab {
 // CG on the normal equations (the body of ctlab_cg); after_iter(k, d_x), when
-// set, runs after iteration k (1-based) and stops the run by returning non-zero
+// set, runs after iteration k (1-based) and stops the run by returning non-zero.
+// When stop is set, ||Ax - b||^2 is carried along (each step lowers it by
+// 2 alpha <r,d> - alpha^2 ||Ad||^2), written to stop->h_fid[k-1] (or not, if
+// NULL), and the run ends after the first step that takes it to f_target or
+// below; stop->done receives the number of steps taken.
+struct CgStop {
+    double f_target;
+    double* h_fid;
+    int done;
+};
 int cg_run(ctlab_recon* R, float* d_x, int iters, int cgm, double* h_resid, hipStream_t s,
-           const std::function<int(int, const float*)>& after_iter);
+           const std::function<int(int, const float*)>& after_iter, CgStop* stop = nullptr);
 }  // namespace ctlab

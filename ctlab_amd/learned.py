@@ -110,7 +110,7 @@ class ScaledCNN:
         self.net = CN.CNN(shape, training=training, params=params)
         self.s = INPUT_SCALE
 
-    def __call__(self, x, out):
+    def __call__(self, x, out, f_ref=None):
         import torch
         xs = x * self.s
         self.net(xs, out=out)
@@ -129,15 +129,27 @@ class DataConsistent:
     follows the noise; the CG steps restore the fit along the directions the
     data constrain most, while the correction stays in the directions they
     constrain little (the artefacts of sparse views).  The safeguard then
-    applies to the result as to any learned step."""
+    applies to the result as to any learned step.
 
-    def __init__(self, net, rc, m=3):
+    With to_fidelity, the number of CG steps is not fixed: CG runs until
+    ||Ax_c - b||^2 is at the reference value f_ref the caller passes (the plain
+    iterate's), at most m_max steps, so that the candidate is compared with the
+    plain iterate at equal data residual.  The stop aims 1e-5 (relative) below
+    f_ref, so that the exact re-evaluation of the safeguard does not reject a
+    candidate on the last digits of the value CG carries along."""
+
+    def __init__(self, net, rc, m=3, to_fidelity=False, m_max=10):
         self.net, self.rc, self.m = net, rc, int(m)
+        self.to_fidelity, self.m_max = bool(to_fidelity), int(m_max)
+        self.last_steps = None
 
-    def __call__(self, x, out):
+    def __call__(self, x, out, f_ref=None):
         self.net(x, out)
-        if self.m > 0:
+        if self.to_fidelity and f_ref is not None:
+            self.last_steps, _ = self.rc.cg_to_fidelity(out, f_ref * (1.0 - 1e-5), self.m_max)
+        elif self.m > 0:
             self.rc.cg(out, self.m)
+            self.last_steps = self.m
 
 
 def noise_energy(b, I0):
@@ -224,9 +236,21 @@ def network_loss(xs, ts, shape, params):
     return float(np.mean(vals))
 
 
-def train(xs, ts, shape, epochs, lr=1e-3, seed=0, log=None):
-    """Adam on 0.5*mean((x + net(x) - phantom)^2) in the scaled units; the last
-    epoch runs at 0.3*lr.  Returns params, per-epoch mean losses, seconds, steps."""
+def learning_rate(lr, schedule, step, total, epoch, epochs):
+    """Step size of Adam step `step` (0-based) of `total`: "step" keeps lr and
+    runs the last epoch at 0.3*lr; "cosine" decays from lr to 0.05*lr along a
+    half cosine over all steps."""
+    if schedule == "cosine":
+        return lr * (0.05 + 0.95 * 0.5 * (1.0 + math.cos(math.pi * step / max(total, 1))))
+    if schedule == "step":
+        return lr * (0.3 if epoch == epochs - 1 and epochs > 1 else 1.0)
+    raise ValueError(f"unknown learning-rate schedule {schedule!r}")
+
+
+def train(xs, ts, shape, epochs, lr=1e-3, seed=0, log=None, schedule="step"):
+    """Adam on 0.5*mean((x + net(x) - phantom)^2) in the scaled units, step
+    sizes by learning_rate(schedule).  Returns params, per-epoch mean losses,
+    seconds, steps."""
     import torch
     net = CN.CNN(shape, training=True, params=CN.init_params(seed))
     s = INPUT_SCALE
@@ -234,11 +258,12 @@ def train(xs, ts, shape, epochs, lr=1e-3, seed=0, log=None):
     order_rng = np.random.default_rng(seed)
     t0 = time.perf_counter()
     steps = 0
+    total = epochs * len(xs)
     for ep in range(epochs):
         order = order_rng.permutation(len(xs))
         ep_loss = 0.0
         for k, i in enumerate(order):
-            cur_lr = lr * (0.3 if ep == epochs - 1 and epochs > 1 else 1.0)
+            cur_lr = learning_rate(lr, schedule, steps, total, ep, epochs)
             loss = net.loss_grad(xs[i] * s, ts[i] * s)
             net.adam_step(cur_lr)
             ep_loss += loss
@@ -306,7 +331,7 @@ def fista_tv_run(rc, L, b_shape_vol, iters, tv_lambda, tv_iters, truth, zsl, dev
                 out, f_out = x, None
                 if tried:
                     f_x = rc.data_fidelity(x)
-                    learned(x, xc)
+                    learned(x, xc, f_ref=f_x)
                     f_c = rc.data_fidelity(xc)
                     f_out = f_x
                     if f_c <= f_x or (noise_level is not None and f_c <= noise_level):
@@ -319,7 +344,7 @@ def fista_tv_run(rc, L, b_shape_vol, iters, tv_lambda, tv_iters, truth, zsl, dev
                     f_out = f_x
             else:
                 RC.tv_prox_into(z, xt, tv_lambda / L, tv_iters, shape, work)
-                learned(xt, xc)
+                learned(xt, xc, f_ref=f_x)
                 f_c = rc.data_fidelity(xc)
                 if f_c < f_x:
                     x.copy_(xc)
@@ -353,7 +378,8 @@ def fista_tv_run(rc, L, b_shape_vol, iters, tv_lambda, tv_iters, truth, zsl, dev
             taken[k + 1] = x.clone()
         hist.append({"iter": k + 1, "t": elapsed, "psnr": psnr(out, truth, zsl) if truth is not None else None,
                      "fidelity": f_out, "learned": acc, "rejected_candidate_psnr": cand_psnr,
-                     "rejected_candidate_fidelity": cand_fid})
+                     "rejected_candidate_fidelity": cand_fid,
+                     "dc_steps": getattr(learned, "last_steps", None) if tried else None})
     return out, hist, taken
 
 

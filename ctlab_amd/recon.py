@@ -53,6 +53,16 @@ class Recon:
                                        _stream_handle(stream)))
         return res[:iters]
 
+    def cg_to_fidelity(self, x, f_target, max_iters, stream=None):
+        """Fletcher-Reeves CG from x until ||Ax - b||^2 <= f_target or max_iters
+        steps; returns (steps taken, ||Ax - b||^2 after each step)."""
+        fid = np.zeros(max(max_iters, 1), dtype=np.float64)
+        done = C.c_int(0)
+        _lib.check(_lib.lib().ctlab_cg_to_fidelity(self.h, _p(x), max_iters, float(f_target),
+                                                    fid.ctypes.data_as(C.POINTER(C.c_double)), C.byref(done),
+                                                    _stream_handle(stream)))
+        return done.value, fid[:done.value]
+
     def data_fidelity(self, x, stream=None):
         v = _lib.lib().ctlab_recon_data_fidelity(self.h, _p(x), _stream_handle(stream))
         _lib.check()

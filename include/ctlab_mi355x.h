@@ -158,6 +158,14 @@ int ctlab_fista_tv(ctlab_recon* recon, float* d_x, int iters, float tv_lambda, i
 /* CG on the normal equations, beta rule cgm = CG_FR..CG_DY (src/DD3_GPU_recon.h:14-19,
  * src/utilities.hpp:1228-1268); h_resid (iters, or NULL) receives ||A^T(b-Ax)||^2. */
 int ctlab_cg(ctlab_recon* recon, float* d_x, int iters, int cgm, double* h_resid, void* stream);
+/* No counterpart in the reference: the same CG (Fletcher-Reeves) from d_x, run
+ * until ||Ax - b||^2 <= f_target or max_iters steps.  The value is carried along
+ * exactly (each step lowers it by 2 alpha <A^T(b - Ax), d> - alpha^2 ||Ad||^2, no
+ * extra projection); h_fid (max_iters, or NULL) receives it after each step and
+ * *iters_done the steps taken.  For callers that compare candidates at equal
+ * data residual (the learned step's data-consistency stage). */
+int ctlab_cg_to_fidelity(ctlab_recon* recon, float* d_x, int max_iters, double f_target, double* h_fid,
+                         int* iters_done, void* stream);
 /* ||Ax - b||^2 (FP64) */
 double ctlab_recon_data_fidelity(ctlab_recon* recon, const float* d_x, void* stream);
 /* d_g = A^T(Ay - b); returns ||Ay - b||^2 */

@@ -107,6 +107,11 @@ def test_plan_recon_batch_argument_checks():
     assert L.ctlab_batch_autotune(None, None, 1.0, None) == 1
     assert L.ctlab_batch_size(None) == 0
     assert L.ctlab_recon_set_tiles(None, 8, 8) == 1
+    done = C.c_int(-1)
+    assert L.ctlab_cg_to_fidelity(None, None, 3, 0.0, None, None, None) == 1
+    assert L.ctlab_cg_to_fidelity(None, None, 3, float("nan"), None, C.byref(done), None) == 1
+    assert L.ctlab_cg_to_fidelity(None, None, 3, 0.0, None, C.byref(done), None) == 1
+    assert _lib.last_error()[0] == 1
 
 
 def test_multi_gpu_split_validation():

@@ -155,3 +155,19 @@ def test_cxx_header_reference_names(tmp_path):
     assert (s, e) == dd.getVolZIdxPair(z, 1, 3, 1.5, 2.0, 4, 7.5, 1.0, 16)
     assert (ps, pe) == dd.getPrjIdxPair(z, 4, 8, 7.5, 1.0, 16, 1.5, 2.0, 4)
     assert threw == 1
+
+
+def test_learning_rate_schedules():
+    """The learned prox's Adam step sizes: the half-cosine decay starts at lr,
+    passes the midpoint of lr and its 0.05 floor halfway and ends on the floor,
+    never rising; the step schedule keeps lr and drops the last epoch to 0.3 lr."""
+    from ctlab_amd import learned as LE
+    lr, T = 1e-3, 600
+    cos = [LE.learning_rate(lr, "cosine", s, T, 0, 1) for s in range(T + 1)]
+    assert cos[0] == pytest.approx(lr) and cos[T] == pytest.approx(0.05 * lr)
+    assert cos[T // 2] == pytest.approx(0.525 * lr)
+    assert all(a >= b for a, b in zip(cos, cos[1:]))
+    assert [LE.learning_rate(lr, "step", 0, T, e, 3) for e in range(3)] == [lr, lr, pytest.approx(0.3 * lr)]
+    assert LE.learning_rate(lr, "step", 0, T, 0, 1) == lr
+    with pytest.raises(ValueError):
+        LE.learning_rate(lr, "linear", 0, T, 0, 1)

@@ -128,3 +128,38 @@ def test_fista_tv_and_cg_match_reference_drivers(cuda):
     f = rc.data_fidelity(xc)
     r = O.proj(g, xc.cpu().numpy()) - b
     assert abs(f - r @ r) < 1e-4 * (r @ r)
+
+
+def test_cg_to_fidelity_stops_at_target(cuda):
+    """CG until ||Ax-b||^2 reaches a target: the iterates of ctlab_cg for the
+    steps it takes (bit for bit), the value it carries along equal to a fresh
+    evaluation and to the FP64 oracle's residual, the cap, and no step when
+    the start already meets the target."""
+    import torch
+    g, x, b = _problem()
+    plan = dd.Plan(g)
+    rc = RC.Recon(plan, osNum=1)
+    rc.set_data(T(b, cuda))
+    f0 = rc.data_fidelity(torch.zeros(g.vol_shape, device=cuda))
+    xs, fs = [], []
+    for k in (1, 2, 3, 4):
+        xk = torch.zeros(g.vol_shape, device=cuda)
+        rc.cg(xk, k)
+        xs.append(xk)
+        fs.append(rc.data_fidelity(xk))
+    assert fs[0] > fs[1] > fs[2] > fs[3]
+    xa = torch.zeros(g.vol_shape, device=cuda)
+    done, fid = rc.cg_to_fidelity(xa, 0.5 * (fs[1] + fs[2]), 10)
+    assert done == 3 and len(fid) == 3
+    assert torch.equal(xa, xs[2])
+    assert np.all(np.diff(fid) < 0)
+    for k in range(3):
+        assert abs(fid[k] - fs[k]) <= 1e-5 * f0, k
+    r = O.proj(g, xa.cpu().numpy()) - b
+    assert abs(fid[-1] - r @ r) < 1e-4 * (r @ r) + 1e-6 * f0
+    xb = torch.zeros(g.vol_shape, device=cuda)
+    done, fid = rc.cg_to_fidelity(xb, 0.0, 4)
+    assert done == 4 and torch.equal(xb, xs[3])
+    xc = xs[1].clone()
+    done, fid = rc.cg_to_fidelity(xc, fs[1] * 1.001, 10)
+    assert done == 0 and len(fid) == 0 and torch.equal(xc, xs[1])
