@@ -186,6 +186,9 @@ def learned_bench(args):
                                    learned_every=5, feedback=False)
     _, h_dca10, _ = LE.fista_tv_run(re_, Le, ge.vol_shape, args.iters, lam, 10, ph, ze, dev, learned=dca,
                                     learned_every=10, feedback=False)
+    # the same step fed back into the iteration (momentum restarted after an accepted step)
+    _, h_dcal10, _ = LE.fista_tv_run(re_, Le, ge.vol_shape, args.iters, lam, 10, ph, ze, dev, learned=dca,
+                                     learned_every=10, feedback=True)
     target = 40.0
     # report only (the weight above is the one chosen without the evaluation phantom): plain
     # FISTA-TV on the evaluation phantom itself for nearby weights and longer runs
@@ -252,11 +255,13 @@ def learned_bench(args):
         "learned_dc_output_every10": summary(h_dc10),
         "learned_dc_adaptive_output_every5": summary(h_dca5),
         "learned_dc_adaptive_output_every10": summary(h_dca10),
+        "learned_dc_adaptive_in_loop_every10": summary(h_dcal10),
         "dc_steps": args.dc_steps, "dc_max": args.dc_max,
         "target_40dB": target_report(h_plain, {"learned_dc_output_every10": h_dc10,
                                                "learned_dc_output_every5": h_dc5,
                                                "learned_dc_adaptive_output_every10": h_dca10,
-                                               "learned_dc_adaptive_output_every5": h_dca5}, target),
+                                               "learned_dc_adaptive_output_every5": h_dca5,
+                                               "learned_dc_adaptive_in_loop_every10": h_dcal10}, target),
         "plain_eval_phantom_sweep": {"iters": args.plain_sweep_iters, "by_lambda": plain_sweep},
         "ref_psnr_db": ref_db,
         "cnn_forward": {"shape": [512, 512, 64], "s": t_fwd, "tflops": rate / 1e12,
@@ -264,7 +269,8 @@ def learned_bench(args):
         "history": {"plain": h_plain, "learned_in_loop": h_learn, "learned_in_loop_every5": h_every,
                     "learned_output_every5": h_out, "learned_output_every5_discrepancy": h_disc,
                     "learned_dc_output_every5": h_dc5, "learned_dc_output_every10": h_dc10,
-                    "learned_dc_adaptive_output_every5": h_dca5, "learned_dc_adaptive_output_every10": h_dca10},
+                    "learned_dc_adaptive_output_every5": h_dca5, "learned_dc_adaptive_output_every10": h_dca10,
+                    "learned_dc_adaptive_in_loop_every10": h_dcal10},
     }
     print(json.dumps(res))
 

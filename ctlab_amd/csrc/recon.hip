@@ -142,11 +142,20 @@ int cg_run(ctlab_recon* R, float* d_x, int iters, int cgm, double* h_resid, hipS
     float* rn = R->d_vol3;
     float* d = R->d_vol4;
     float* w = R->d_vol;
-    // r = A^T (b - A x) = -gradient
-    double f = ctlab_recon_gradient(R, d_x, r, stream);
-    if (std::isnan(f)) return ctlab_last_error();
-    if (stop) stop->done = 0;
-    const int steps = (stop && f <= stop->f_target) ? 0 : iters;   // already there: no step
+    // r = A^T (b - A x) = -gradient; with a stop target that x already meets,
+    // only the residual is formed and no step is taken
+    double f;
+    if (stop) {
+        stop->done = 0;
+        f = residual(R, d_x, s, true);
+        if (std::isnan(f)) return ctlab_last_error();
+        if (f <= stop->f_target) return chk(hipStreamSynchronize(s), "cg") ? ctlab_last_error() : CTLAB_OK;
+        if (plan_back(R->full, R->d_prj, r, s)) return ctlab_last_error();
+    } else {
+        f = ctlab_recon_gradient(R, d_x, r, stream);
+        if (std::isnan(f)) return ctlab_last_error();
+    }
+    const int steps = iters;
     if (chk(rk_axpby(0.0f, r, -1.0f, r, n, s), "negate")) return ctlab_last_error();
     hipMemcpyAsync(d, r, n * sizeof(float), hipMemcpyDeviceToDevice, s);
     for (int it = 0; it < steps; ++it) {
