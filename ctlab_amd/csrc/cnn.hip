@@ -6,6 +6,7 @@
 #include <vector>
 
 #include "../../include/ctlab_mi355x.h"
+#include "cnn_tiles.hpp"
 #include "plan.hpp"
 #include "status.hpp"
 
@@ -109,11 +110,11 @@ int forward(ctlab_cnn* N, const float* x, float* out, const float* target, hipSt
         const __bf16* in = N->training ? N->a_cl[l - 1] : N->a_cl[(l - 1) & 1];
         __bf16* ocl = N->training ? N->a_cl[l] : N->a_cl[l & 1];
         __bf16* ocm = N->training ? N->a_cm[l] : nullptr;
-        hipLaunchKernelGGL(k_conv32, dim3(blocks(N->nvox, 256)), dim3(256), 0, s, in, N->wb + (l - 1) * kW32,
+        hipLaunchKernelGGL(k_conv32, dim3(conv32_grid(XN, YN, ZN)), dim3(256), 0, s, in, N->wb + (l - 1) * kW32,
                            N->p + kOffW[l] + kLenW[l], (const __bf16*)nullptr, ocl, ocm, XN, YN, ZN, 1);
     }
     const __bf16* a4 = N->training ? N->a_cl[3] : N->a_cl[1];
-    hipLaunchKernelGGL(k_conv_last, dim3(blocks(N->nvox, 256)), dim3(256), 0, s, a4, N->wr + 27 * CH,
+    hipLaunchKernelGGL(k_conv_last, dim3((unsigned)conv_last_tiles(XN, YN, ZN)), dim3(256), 0, s, a4, N->wr + 27 * CH,
                        N->p + kOffW[4] + kLenW[4], x, out, target, N->g5, (float)(1.0 / (double)N->nvox),
                        target ? N->lpart : (double*)nullptr, XN, YN, ZN);
     return chk(hipGetLastError(), "cnn forward");
@@ -159,7 +160,7 @@ int backward(ctlab_cnn* N, const float* x, hipStream_t s)
                            kVecChunk, N->part);
         reduce_chunks(N, N->part, (int)N->nvchunks, CH, G + kOffW[l] + kLenW[l], s);
         // input gradient through layer l+1, masked by the ReLU of layer l
-        hipLaunchKernelGGL(k_conv32, dim3(blocks(n, 256)), dim3(256), 0, s, N->g_cl[cur], N->wt + (l - 1) * kW32,
+        hipLaunchKernelGGL(k_conv32, dim3(conv32_grid(XN, YN, ZN)), dim3(256), 0, s, N->g_cl[cur], N->wt + (l - 1) * kW32,
                            (const float*)nullptr, N->a_cl[l - 1], N->g_cl[cur ^ 1], N->g_cm[cur ^ 1], XN, YN, ZN, 0);
         cur ^= 1;
     }
@@ -204,7 +205,7 @@ ctlab_cnn* ctlab_cnn_create(int XN, int YN, int ZN, int training, int gpunum)
               !chk(hipMalloc(&N->wt, 3 * kW32 * sizeof(__bf16)), "hipMalloc") &&
               !chk(hipMalloc(&N->out, N->nvox * sizeof(float)), "hipMalloc") &&
               !chk(hipMalloc(&N->g5, N->nvox * sizeof(float)), "hipMalloc") &&
-              !chk(hipMalloc(&N->lpart, blocks(N->nvox, 256) * sizeof(double)), "hipMalloc");
+              !chk(hipMalloc(&N->lpart, conv_last_tiles(XN, YN, ZN) * sizeof(double)), "hipMalloc");
     const int nact = training ? 4 : 2;
     for (int i = 0; ok && i < nact; ++i) ok = !chk(hipMalloc(&N->a_cl[i], A), "hipMalloc(activations)");
     if (training) {
@@ -266,7 +267,7 @@ double ctlab_cnn_loss_grad(ctlab_cnn* N, const float* d_x, const float* d_target
     DeviceGuard dg(N->device);
     hipStream_t s = (hipStream_t)stream;
     if (forward(N, d_x, d_out ? d_out : N->out, d_target, s) || backward(N, d_x, s)) return NAN;
-    const long nb = blocks(N->nvox, 256);
+    const long nb = conv_last_tiles(N->XN, N->YN, N->ZN);       // one partial per k_conv_last block
     std::vector<double> lp(nb);
     if (chk(hipMemcpyAsync(lp.data(), N->lpart, nb * sizeof(double), hipMemcpyDeviceToHost, s), "D2H")) return NAN;
     if (h_grad && chk(hipMemcpyAsync(h_grad, N->g, kNParams * sizeof(float), hipMemcpyDeviceToHost, s), "D2H"))

@@ -11,11 +11,14 @@
 // one MFMA per (16 voxels, 16 output channels, tap) with K = the 32 input
 // channels.  The input gradient runs the same kernel with flipped, transposed
 // weights and the ReLU mask fused into its epilogue; the weight gradient runs
-// MFMAs over K = voxels on the channel-major copies.  The thin first and last
-// layers run on the VALU.
+// MFMAs over K = voxels on the channel-major copies.  The 32 -> 1 output
+// layer runs on MFMA with the (dy, dx) taps as the N dimension; the 1 -> 32
+// first layer and the thin gradient kernels run on the VALU.
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+
+#include "cnn_tiles.hpp"
 
 namespace ctlab {
 
@@ -41,79 +44,94 @@ __device__ __forceinline__ bool tap_ok(int t, int y, int x, int z, int YN, int X
 __device__ __forceinline__ float bf(const __bf16 v) { return (float)v; }
 
 // ---------------------------------------------------------------------------
-// 32 -> 32 convolution, MFMA.  Block = 4 wavefronts x 64 voxels.
+// 32 -> 32 convolution, MFMA.  Block = 4 wavefronts, output tile = 8 x-columns
+// x 32 z of one y row (256 voxels; each wavefront two columns, 4 blocks of 16
+// voxels along z).
 //   out = act(conv(in; w) + bias) ; act: relu (relu=1) or identity
 //   mask (optional, channels-last): out *= (mask > 0)   [input-gradient pass]
 // Writes channels-last and (optionally) channel-major copies.
-// The weights pass through LDS one dy plane (9 taps, 18 KB) at a time, so
-// that several blocks fit on a CU, and each tap's activation loads are issued
-// while the previous tap's MFMAs run.
+// Per dy plane, the tile's input brick (10 columns x 34 z of row y+dy, all 32
+// channels, zero outside the volume) and the plane's 9 taps of weights are
+// staged in LDS (21.8 + 18.4 KB: 4 blocks per CU); the 9 taps then read their
+// MFMA operands from LDS, so each input voxel is fetched from L2 about 1.3
+// times per plane instead of 9.  Registers are held to 128 (4 wavefronts per
+// SIMD, what the LDS allows): 136 registers and 3 wavefronts ran 14 % slower.  Tiles are dealt to the 8 XCDs in contiguous
+// ranges (y rows of one range share their input rows in that XCD's L2).
 // ---------------------------------------------------------------------------
-constexpr int kConvMB = 4;           // 16-voxel blocks per wavefront
+constexpr int kBX = kTX + 2, kBZ = kTZ + 2;         // input brick of one plane
+constexpr int kBrick = kBX * kBZ;                   // 340 voxels
 
-__global__ void __launch_bounds__(256)
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4)))
 k_conv32(const __bf16* __restrict__ in, const __bf16* __restrict__ w, const float* __restrict__ bias,
          const __bf16* __restrict__ mask, __bf16* __restrict__ out_cl, __bf16* __restrict__ out_cm,
          int XN, int YN, int ZN, int relu)
 {
-    __shared__ bf16x8 sW[9 * CH * CH / 8];         // 18,432 B: the 9 taps of one dy plane
-    const long nvox = (long)XN * YN * ZN;
+    __shared__ bf16x8 sW[9 * CH * CH / 8];          // 18,432 B: the 9 taps of one dy plane
+    __shared__ bf16x8 sA[kBrick * CH / 8];          // 21,760 B: the input brick of one plane
+    const long ntiles = conv32_tiles(XN, YN, ZN);
+    // contiguous tile ranges per XCD (blocks are dealt to XCDs round-robin); the
+    // grid is 8 * ceil(ntiles / 8), so (xcd, j) covers [0, 8 * per) once
+    const long per = (long)gridDim.x >> 3;
+    const long tile = (long)(blockIdx.x & 7) * per + (blockIdx.x >> 3);
+    if (tile >= ntiles) return;                     // whole block: before any barrier
+    const int ntz = (ZN + kTZ - 1) / kTZ, ntx = (XN + kTX - 1) / kTX;
+    const int zt = (int)(tile % ntz);
+    const long r = tile / ntz;
+    const int xt = (int)(r % ntx);
+    const int y = (int)(r / ntx);
+    const int x0 = xt * kTX, z0 = zt * kTZ;
 
     const int lane = threadIdx.x & 63;
     const int wv = threadIdx.x >> 6;
     const int m = lane & 15, kq = lane >> 4;
-    const long vbase = ((long)blockIdx.x * 4 + wv) * (16 * kConvMB);
+    // block b of this wavefront: column 2*wv + (b >> 1), z 16*(b & 1) + m; its
+    // centre voxel in the brick is (column + 1, z + 1)
+    int pb[4];
+#pragma unroll
+    for (int b = 0; b < 4; ++b) pb[b] = (2 * wv + (b >> 1) + 1) * kBZ + 16 * (b & 1) + m + 1;
 
-    int vy[kConvMB], vx[kConvMB], vz[kConvMB];
-    long vv[kConvMB];
+    f32x4 acc[4][2];
 #pragma unroll
-    for (int b = 0; b < kConvMB; ++b) {
-        const long v = vbase + b * 16 + m;
-        vv[b] = v;
-        const long vc = v < nvox ? v : 0;
-        vz[b] = (int)(vc % ZN);
-        const long xy = vc / ZN;
-        vx[b] = (int)(xy % XN);
-        vy[b] = (int)(xy / XN);
-    }
-    f32x4 acc[kConvMB][2];
-#pragma unroll
-    for (int b = 0; b < kConvMB; ++b) acc[b][0] = acc[b][1] = (f32x4){0.f, 0.f, 0.f, 0.f};
+    for (int b = 0; b < 4; ++b) acc[b][0] = acc[b][1] = (f32x4){0.f, 0.f, 0.f, 0.f};
 
-    auto load_a = [&](int t, bf16x8 (&a)[kConvMB]) {
-        const int off = tap_off(t, XN, ZN);
-#pragma unroll
-        for (int b = 0; b < kConvMB; ++b) {
-            a[b] = (bf16x8){0, 0, 0, 0, 0, 0, 0, 0};
-            if (vv[b] < nvox && tap_ok(t, vy[b], vx[b], vz[b], YN, XN, ZN))
-                a[b] = *(const bf16x8*)(in + (vv[b] + off) * CH + 8 * kq);
-        }
-    };
-    bf16x8 a_cur[kConvMB], a_nxt[kConvMB];
-    load_a(0, a_cur);
+    const bf16x8 zero8 = (bf16x8){0, 0, 0, 0, 0, 0, 0, 0};
     for (int plane = 0; plane < 3; ++plane) {
-        __syncthreads();   // the previous plane's weights are no longer read
+        __syncthreads();   // the previous plane's operands are no longer read
         for (int i = threadIdx.x; i < 9 * CH * CH / 8; i += 256)
             sW[i] = ((const bf16x8*)w)[plane * 9 * CH * CH / 8 + i];
+        const int yy = y + plane - 1;
+        const bool row_in = (unsigned)yy < (unsigned)YN;
+        for (int i = threadIdx.x; i < kBrick * CH / 8; i += 256) {
+            const int p = i >> 2, q = i & 3;
+            const int gx = x0 - 1 + p / kBZ, gz = z0 - 1 + p % kBZ;
+            bf16x8 v = zero8;
+            if (row_in && (unsigned)gx < (unsigned)XN && (unsigned)gz < (unsigned)ZN)
+                v = ((const bf16x8*)in)[(((long)yy * XN + gx) * ZN + gz) * (CH / 8) + q];
+            sA[i] = v;
+        }
         __syncthreads();
+#pragma unroll
         for (int tt = 0; tt < 9; ++tt) {
-            const int t = plane * 9 + tt;
-            if (t + 1 < 27) load_a(t + 1, a_nxt);
+            const int shift = (tt / 3 - 1) * kBZ + (tt % 3 - 1);      // (dx, dz) in the brick
             const bf16x8 b0 = sW[((tt * CH) + m) * (CH / 8) + kq];
             const bf16x8 b1 = sW[((tt * CH) + 16 + m) * (CH / 8) + kq];
 #pragma unroll
-            for (int b = 0; b < kConvMB; ++b) {
-                acc[b][0] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a_cur[b], b0, acc[b][0], 0, 0, 0);
-                acc[b][1] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a_cur[b], b1, acc[b][1], 0, 0, 0);
+            for (int b = 0; b < 4; ++b) {
+                const bf16x8 a = sA[(pb[b] + shift) * (CH / 8) + kq];
+                acc[b][0] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b0, acc[b][0], 0, 0, 0);
+                acc[b][1] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b1, acc[b][1], 0, 0, 0);
             }
-#pragma unroll
-            for (int b = 0; b < kConvMB; ++b) a_cur[b] = a_nxt[b];
         }
     }
 
-    // D fragment: lane holds rows 4*kq + i (voxels), column m (output channel)
+    // D fragment: lane holds rows 4*kq + i (voxels z), column m (output channel)
+    const long nvox = (long)XN * YN * ZN;
 #pragma unroll
-    for (int b = 0; b < kConvMB; ++b) {
+    for (int b = 0; b < 4; ++b) {
+        const int x = x0 + 2 * wv + (b >> 1);
+        const int zb = z0 + 16 * (b & 1) + 4 * kq;
+        if (x >= XN) continue;
+        const long v0 = ((long)y * XN + x) * ZN + zb;
 #pragma unroll
         for (int nb = 0; nb < 2; ++nb) {
             const int co = nb * 16 + m;
@@ -121,21 +139,21 @@ k_conv32(const __bf16* __restrict__ in, const __bf16* __restrict__ w, const floa
             __bf16 o[4];
 #pragma unroll
             for (int i = 0; i < 4; ++i) {
-                const long v = vbase + b * 16 + 4 * kq + i;
-                float r = acc[b][nb][i] + bco;
-                if (relu) r = fmaxf(r, 0.0f);
-                if (mask && v < nvox && !(bf(mask[v * CH + co]) > 0.0f)) r = 0.0f;
-                o[i] = (__bf16)r;
-                if (v < nvox) out_cl[v * CH + co] = o[i];
+                const long v = v0 + i;
+                const bool ok = zb + i < ZN;
+                float rr = acc[b][nb][i] + bco;
+                if (relu) rr = fmaxf(rr, 0.0f);
+                if (mask && ok && !(bf(mask[v * CH + co]) > 0.0f)) rr = 0.0f;
+                o[i] = (__bf16)rr;
+                if (ok) out_cl[v * CH + co] = o[i];
             }
             if (out_cm) {
-                const long v0 = vbase + b * 16 + 4 * kq;
-                if (v0 + 3 < nvox && (nvox & 3) == 0) {   // 8-byte aligned vector store
+                if (zb + 3 < ZN && (ZN & 3) == 0) {     // v0 is a multiple of 4: 8-byte aligned vector store
                     typedef __bf16 bf16x4 __attribute__((ext_vector_type(4)));
                     *(bf16x4*)(out_cm + (long)co * nvox + v0) = (bf16x4){o[0], o[1], o[2], o[3]};
                 } else {
                     for (int i = 0; i < 4; ++i)
-                        if (v0 + i < nvox) out_cm[(long)co * nvox + v0 + i] = o[i];
+                        if (zb + i < ZN) out_cm[(long)co * nvox + v0 + i] = o[i];
                 }
             }
         }
@@ -181,48 +199,88 @@ k_conv_first(const float* __restrict__ x, const float* __restrict__ w1, const fl
 }
 
 // ---------------------------------------------------------------------------
-// last layer 32 -> 1 (VALU) + residual: out = x + conv(a4; w5) + b5
-// (w5 fp32 [27][32]); with target != null also g5 = (out - target) * gscale
-// and the squared residual per block (FP64) for the loss.
+// last layer 32 -> 1, MFMA: out = x + conv(a4; w5) + b5; training forward
+// also writes g5 = (out - target) * gscale and the block's sum of squares.
+// Block = 4 wavefronts, output tile = 8 x-columns x 4 y-rows x 32 z.  Each of
+// the tile's 10 x 6 input columns (with the halo) and 16-voxel z block gives,
+// in one MFMA chain over dz (K = 32 channels each), its shares of the 9 output
+// columns it reaches: B column n = (dy, dx) holds the weights of tap
+// (dy, dx, dz), so D[z][n] is input column c's share of output column
+// c - (dy, dx).  Every share has its own LDS slot (n, output column, z), and
+// each output adds its 9 shares in a fixed order (deterministic).  w5 holds
+// the BF16-rounded weights as FP32, so the B operand is exact.
 // ---------------------------------------------------------------------------
 __global__ void __launch_bounds__(256)
 k_conv_last(const __bf16* __restrict__ in, const float* __restrict__ w5, const float* __restrict__ b5,
             const float* __restrict__ x, float* __restrict__ out, const float* __restrict__ target,
             float* __restrict__ g5, float gscale, double* __restrict__ part, int XN, int YN, int ZN)
 {
-    __shared__ float sW[27 * CH];
+    constexpr int kCols = kLX * kLY;                        // 32 output columns
+    constexpr int kIX = kLX + 2, kIY = kLY + 2;             // 10 x 6 input columns
+    __shared__ f32x4 sD[9 * kCols * kTZ / 4];               // 36,864 B: shares (n, column, z)
     __shared__ double sred[4];
-    for (int i = threadIdx.x; i < 27 * CH; i += 256) sW[i] = w5[i];
-    __syncthreads();
-    const long nvox = (long)XN * YN * ZN;
-    const long v = blockIdx.x * 256L + threadIdx.x;
-    double r2 = 0.0;
-    if (v < nvox) {
-        const int z = (int)(v % ZN);
-        const long xy = v / ZN;
-        const int xx = (int)(xy % XN), y = (int)(xy / XN);
-        float acc = b5[0];
-        for (int t = 0; t < 27; ++t) {
-            if (!tap_ok(t, y, xx, z, YN, XN, ZN)) continue;
-            const bf16x8* a = (const bf16x8*)(in + (v + tap_off(t, XN, ZN)) * CH);
+    const int ntz = (ZN + kTZ - 1) / kTZ, ntx = (XN + kLX - 1) / kLX;
+    const int zt = (int)(blockIdx.x % ntz);
+    const int r = (int)(blockIdx.x / ntz);
+    const int x0 = (r % ntx) * kLX, y0 = (r / ntx) * kLY, z0 = zt * kTZ;
+
+    const int lane = threadIdx.x & 63;
+    const int wv = threadIdx.x >> 6;
+    const int m = lane & 15, kq = lane >> 4;
+    // B fragments (lane: channels 8kq..8kq+7 of column n = m), one per dz
+    bf16x8 B[3];
+    const int mw = m < 9 ? m : 0;                           // columns 9..15 are zero
 #pragma unroll
-            for (int q = 0; q < CH / 8; ++q) {
-                const bf16x8 av = a[q];
+    for (int dz = 0; dz < 3; ++dz) {
 #pragma unroll
-                for (int e = 0; e < 8; ++e) acc += sW[t * CH + 8 * q + e] * (float)av[e];
-            }
+        for (int e = 0; e < 8; ++e) {
+            const float wv5 = w5[(3 * mw + dz) * CH + 8 * kq + e];
+            B[dz][e] = (__bf16)(m < 9 ? wv5 : 0.0f);
         }
-        const float o = x[v] + acc;
+    }
+    const int nzb = ZN - z0 > 16 ? 2 : 1;                   // z blocks of 16 that hold voxels
+    for (int it = wv; it < kIX * kIY * nzb; it += 4) {      // wave-uniform trip count
+        const int c = it / nzb, zb = it - c * nzb;
+        const int iy = c / kIX, ix = c - iy * kIX;
+        const int gy = y0 - 1 + iy, gx = x0 - 1 + ix;
+        const bool col_ok = (unsigned)gy < (unsigned)YN && (unsigned)gx < (unsigned)XN;
+        const long cbase = ((long)gy * XN + gx) * ZN;
+        f32x4 acc = (f32x4){0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int dz = 0; dz < 3; ++dz) {
+            const int z = z0 + 16 * zb + m + dz - 1;
+            bf16x8 a = (bf16x8){0, 0, 0, 0, 0, 0, 0, 0};
+            if (col_ok && (unsigned)z < (unsigned)ZN) a = *(const bf16x8*)(in + (cbase + z) * CH + 8 * kq);
+            acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, B[dz], acc, 0, 0, 0);
+        }
+        // D: lane holds rows z = 16*zb + 4*kq + i of column n = m
+        const int oy = iy - m / 3, ox = ix - m % 3;             // output column c - (dy, dx)
+        if (m < 9 && (unsigned)oy < (unsigned)kLY && (unsigned)ox < (unsigned)kLX)
+            sD[((m * kCols + oy * kLX + ox) * kTZ + 16 * zb + 4 * kq) >> 2] = acc;
+    }
+    __syncthreads();
+    const float* D = (const float*)sD;
+    const float bias = b5[0];
+    double r2 = 0.0;
+    for (int j = threadIdx.x; j < kCols * kTZ; j += 256) {
+        const int col = j / kTZ, zz = j - col * kTZ;
+        const int gy = y0 + col / kLX, gx = x0 + col % kLX, gz = z0 + zz;
+        if (gy >= YN || gx >= XN || gz >= ZN) continue;
+        float sum = bias;
+#pragma unroll
+        for (int n = 0; n < 9; ++n) sum += D[(n * kCols + col) * kTZ + zz];
+        const long v = ((long)gy * XN + gx) * ZN + gz;
+        const float o = x[v] + sum;
         out[v] = o;
         if (target) {
-            const float r = o - target[v];
-            g5[v] = r * gscale;
-            r2 = (double)r * r;
+            const float rr = o - target[v];
+            g5[v] = rr * gscale;
+            r2 += (double)rr * rr;
         }
     }
     if (part) {
-        for (int s = 32; s > 0; s >>= 1) r2 += __shfl_xor(r2, s, 64);
-        if ((threadIdx.x & 63) == 0) sred[threadIdx.x >> 6] = r2;
+        for (int sh = 32; sh > 0; sh >>= 1) r2 += __shfl_xor(r2, sh, 64);
+        if (lane == 0) sred[wv] = r2;
         __syncthreads();
         if (threadIdx.x == 0) part[blockIdx.x] = sred[0] + sred[1] + sred[2] + sred[3];
     }

@@ -1,0 +1,27 @@
+// Output tile of the 32 -> 32 convolution kernel and its launch grid (shared by
+// the kernel and the host code that launches it).
+#pragma once
+#include <hip/hip_runtime.h>
+
+namespace ctlab {
+constexpr int kTX = 8, kTZ = 32;                    // 8 x-columns x 32 z of one y row
+
+__host__ __device__ inline long conv32_tiles(int XN, int YN, int ZN)
+{
+    return (long)YN * ((XN + kTX - 1) / kTX) * ((ZN + kTZ - 1) / kTZ);
+}
+
+// tiles rounded up to a multiple of the 8 XCDs (the kernel deals contiguous
+// tile ranges to the XCDs and drops the blocks past the last tile)
+inline unsigned conv32_grid(int XN, int YN, int ZN)
+{
+    return (unsigned)((conv32_tiles(XN, YN, ZN) + 7) / 8 * 8);
+}
+// output layer (32 -> 1): 8 x-columns x 4 y-rows x 32 z per block
+constexpr int kLX = 8, kLY = 4;
+
+inline long conv_last_tiles(int XN, int YN, int ZN)
+{
+    return (long)((YN + kLY - 1) / kLY) * ((XN + kLX - 1) / kLX) * ((ZN + kTZ - 1) / kTZ);
+}
+}  // namespace ctlab
