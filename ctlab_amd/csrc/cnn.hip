@@ -103,7 +103,7 @@ int forward(ctlab_cnn* N, const float* x, float* out, const float* target, hipSt
     const int XN = N->XN, YN = N->YN, ZN = N->ZN;
     const bool keep_cm = N->training;
     // layer 1
-    hipLaunchKernelGGL(k_conv_first, dim3(blocks(N->nvox, 256)), dim3(256), 0, s, x, N->wr, N->p + kOffW[0] + kLenW[0],
+    hipLaunchKernelGGL(k_conv_first, dim3(conv32_grid(XN, YN, ZN)), dim3(256), 0, s, x, N->wr, N->p + kOffW[0] + kLenW[0],
                        N->a_cl[0], keep_cm ? N->a_cm[0] : (__bf16*)nullptr, XN, YN, ZN);
     // layers 2..4 (inference ping-pongs between a_cl[0] and a_cl[1])
     for (int l = 1; l <= 3; ++l) {

@@ -161,28 +161,51 @@ k_conv32(const __bf16* __restrict__ in, const __bf16* __restrict__ w, const floa
 }
 
 // ---------------------------------------------------------------------------
-// first layer 1 -> 32 (VALU): out = relu(conv(x; w1) + b1); w1 fp32 [27][32]
+// first layer 1 -> 32 (VALU): out = relu(conv(x; w1) + b1); w1 fp32 [27][32].
+// Same tiles as k_conv32 (8 x-columns x 32 z of one y row, thread = voxel):
+// the tile's input brick (3 rows x 10 columns x 34 z of x, zero outside the
+// volume) is staged in LDS with coalesced loads, so the 27 taps read their
+// operand from LDS instead of waiting on one global load each.
 // ---------------------------------------------------------------------------
 __global__ void __launch_bounds__(256)
 k_conv_first(const float* __restrict__ x, const float* __restrict__ w1, const float* __restrict__ b1,
              __bf16* __restrict__ out_cl, __bf16* __restrict__ out_cm, int XN, int YN, int ZN)
 {
     __shared__ float sW[27 * CH + CH];
+    __shared__ float sX[3 * kBrick];                        // 4,080 B
+    const long ntiles = conv32_tiles(XN, YN, ZN);
+    const long per = (long)gridDim.x >> 3;
+    const long tile = (long)(blockIdx.x & 7) * per + (blockIdx.x >> 3);
+    if (tile >= ntiles) return;                             // whole block: before any barrier
+    const int ntz = (ZN + kTZ - 1) / kTZ, ntx = (XN + kTX - 1) / kTX;
+    const int zt = (int)(tile % ntz);
+    const long r = tile / ntz;
+    const int x0 = (int)(r % ntx) * kTX, y = (int)(r / ntx), z0 = zt * kTZ;
+
     for (int i = threadIdx.x; i < 27 * CH; i += 256) sW[i] = w1[i];
     for (int i = threadIdx.x; i < CH; i += 256) sW[27 * CH + i] = b1[i];
+    for (int i = threadIdx.x; i < 3 * kBrick; i += 256) {
+        const int pl = i / kBrick, p = i - pl * kBrick;
+        const int gy = y + pl - 1, gx = x0 - 1 + p / kBZ, gz = z0 - 1 + p % kBZ;
+        float v = 0.0f;
+        if ((unsigned)gy < (unsigned)YN && (unsigned)gx < (unsigned)XN && (unsigned)gz < (unsigned)ZN)
+            v = x[((long)gy * XN + gx) * ZN + gz];
+        sX[i] = v;
+    }
     __syncthreads();
+    const int col = threadIdx.x >> 5, zz = threadIdx.x & 31;
+    const int xx = x0 + col, z = z0 + zz;
+    if (xx >= XN || z >= ZN) return;                        // no barrier follows
     const long nvox = (long)XN * YN * ZN;
-    const long v = blockIdx.x * 256L + threadIdx.x;
-    if (v >= nvox) return;
-    const int z = (int)(v % ZN);
-    const long xy = v / ZN;
-    const int xx = (int)(xy % XN), y = (int)(xy / XN);
+    const long v = ((long)y * XN + xx) * ZN + z;
+    const int base = kBrick + (col + 1) * kBZ + zz + 1;     // centre of the brick, row y
     float acc[CH];
 #pragma unroll
     for (int c = 0; c < CH; ++c) acc[c] = sW[27 * CH + c];
+#pragma unroll 1
     for (int t = 0; t < 27; ++t) {
-        if (!tap_ok(t, y, xx, z, YN, XN, ZN)) continue;
-        const float a = x[v + tap_off(t, XN, ZN)];
+        const int dy = t / 9 - 1, dx = (t / 3) % 3 - 1, dz = t % 3 - 1;
+        const float a = sX[base + dy * kBrick + dx * kBZ + dz];
 #pragma unroll
         for (int c = 0; c < CH; ++c) acc[c] += sW[t * CH + c] * a;
     }
@@ -239,24 +262,41 @@ k_conv_last(const __bf16* __restrict__ in, const float* __restrict__ w5, const f
         }
     }
     const int nzb = ZN - z0 > 16 ? 2 : 1;                   // z blocks of 16 that hold voxels
-    for (int it = wv; it < kIX * kIY * nzb; it += 4) {      // wave-uniform trip count
+    const int nit = kIX * kIY * nzb;
+    // item it = (input column c, z block zb); a wavefront takes it = wv, wv + 4, ...
+    // and keeps the loads of the next two items in flight
+    auto load_item = [&](int it, bf16x8 (&a)[3]) {
         const int c = it / nzb, zb = it - c * nzb;
         const int iy = c / kIX, ix = c - iy * kIX;
         const int gy = y0 - 1 + iy, gx = x0 - 1 + ix;
-        const bool col_ok = (unsigned)gy < (unsigned)YN && (unsigned)gx < (unsigned)XN;
+        const bool ok = it < nit && (unsigned)gy < (unsigned)YN && (unsigned)gx < (unsigned)XN;
         const long cbase = ((long)gy * XN + gx) * ZN;
-        f32x4 acc = (f32x4){0.f, 0.f, 0.f, 0.f};
 #pragma unroll
         for (int dz = 0; dz < 3; ++dz) {
             const int z = z0 + 16 * zb + m + dz - 1;
-            bf16x8 a = (bf16x8){0, 0, 0, 0, 0, 0, 0, 0};
-            if (col_ok && (unsigned)z < (unsigned)ZN) a = *(const bf16x8*)(in + (cbase + z) * CH + 8 * kq);
-            acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, B[dz], acc, 0, 0, 0);
+            a[dz] = (bf16x8){0, 0, 0, 0, 0, 0, 0, 0};
+            if (ok && (unsigned)z < (unsigned)ZN) a[dz] = *(const bf16x8*)(in + (cbase + z) * CH + 8 * kq);
         }
+    };
+    bf16x8 a0[3], a1[3], a2[3];
+    load_item(wv, a0);
+    load_item(wv + 4, a1);
+    for (int it = wv; it < nit; it += 4) {                  // wave-uniform trip count
+        load_item(it + 8, a2);
+        f32x4 acc = (f32x4){0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int dz = 0; dz < 3; ++dz) acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a0[dz], B[dz], acc, 0, 0, 0);
         // D: lane holds rows z = 16*zb + 4*kq + i of column n = m
+        const int c = it / nzb, zb = it - c * nzb;
+        const int iy = c / kIX, ix = c - iy * kIX;
         const int oy = iy - m / 3, ox = ix - m % 3;             // output column c - (dy, dx)
         if (m < 9 && (unsigned)oy < (unsigned)kLY && (unsigned)ox < (unsigned)kLX)
             sD[((m * kCols + oy * kLX + ox) * kTZ + 16 * zb + 4 * kq) >> 2] = acc;
+#pragma unroll
+        for (int dz = 0; dz < 3; ++dz) {
+            a0[dz] = a1[dz];
+            a1[dz] = a2[dz];
+        }
     }
     __syncthreads();
     const float* D = (const float*)sD;
