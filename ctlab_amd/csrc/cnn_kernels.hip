@@ -402,52 +402,87 @@ k_wgrad32(const __bf16* __restrict__ g_cm, const __bf16* __restrict__ a_cm, floa
             for (int j = 0; j < 2; ++j) acc[d][i][j] = (f32x4){0.f, 0.f, 0.f, 0.f};
     const long shift = (long)dx * ZN + (long)dy * XN * ZN;
     const uint32_t* a32 = (const uint32_t*)a_cm;
-    for (long v0 = c0; v0 < c1; v0 += 32) {
-        const long vr = v0 + 8 * kq;                 // this lane's run of 8 voxels
-        const int z0 = (int)(vr % ZN);
-        const long xy = vr / ZN;
-        const int xx = (int)(xy % XN), y = (int)(xy / XN);
-        const bool ok = vr < nvox && (unsigned)(y + dy) < (unsigned)YN && (unsigned)(xx + dx) < (unsigned)XN;
-        bf16x8 A0 = *(const bf16x8*)(g_cm + (long)m * nvox + (vr < nvox ? vr : 0));
-        bf16x8 A1 = *(const bf16x8*)(g_cm + (long)(16 + m) * nvox + (vr < nvox ? vr : 0));
-        if (vr >= nvox) A0 = A1 = (bf16x8){0, 0, 0, 0, 0, 0, 0, 0};
+    // operands of one 32-voxel step; the next step's are loaded while the
+    // current step's 12 MFMAs run.  This lane's run of 8 voxels starts at
+    // vr = v0 + 8*kq, whose (z, x, y) are stepped, not divided (ZN % 8 == 0:
+    // a run never crosses a column)
+    struct Ops {
+        bf16x8 A0, A1;
+        u32x4 blk[2];
+        uint32_t prev[2], next[2];
+        int z0;
+    };
+    long vr = c0 + 8 * kq;
+    int rz, rx, ry;
+    {
+        const long vc = vr < nvox ? vr : 0;
+        rz = (int)(vc % ZN);
+        const long xy = vc / ZN;
+        rx = (int)(xy % XN);
+        ry = (int)(xy / XN);
+    }
+    auto load = [&](Ops& o) {
+        const bool in = vr < nvox;
+        const bool ok = in && (unsigned)(ry + dy) < (unsigned)YN && (unsigned)(rx + dx) < (unsigned)XN;
+        o.z0 = rz;
+        o.A0 = o.A1 = (bf16x8){0, 0, 0, 0, 0, 0, 0, 0};
+        if (in) {
+            o.A0 = *(const bf16x8*)(g_cm + (long)m * nvox + vr);
+            o.A1 = *(const bf16x8*)(g_cm + (long)(16 + m) * nvox + vr);
+        }
 #pragma unroll
         for (int nb = 0; nb < 2; ++nb) {
-            const int ci = nb * 16 + m;
-            u32x4 blk = (u32x4){0u, 0u, 0u, 0u};
-            uint32_t prev = 0u, next = 0u;
+            o.blk[nb] = (u32x4){0u, 0u, 0u, 0u};
+            o.prev[nb] = o.next[nb] = 0u;
             if (ok) {
-                const long e = (long)ci * nvox + vr + shift;     // 8-aligned element index
-                blk = *(const u32x4*)(a_cm + e);
+                const long e = (long)(nb * 16 + m) * nvox + vr + shift;     // 8-aligned element index
+                o.blk[nb] = *(const u32x4*)(a_cm + e);
                 const long ep = e - 2 >= 0 ? e - 2 : 0;
                 const long en = e + 8 <= (long)CH * nvox - 2 ? e + 8 : e;
-                prev = a32[ep >> 1];
-                next = a32[en >> 1];
+                o.prev[nb] = a32[ep >> 1];
+                o.next[nb] = a32[en >> 1];
             }
+        }
+        // advance the run by 32 voxels
+        vr += 32;
+        rz += 32;
+        while (rz >= ZN) {
+            rz -= ZN;
+            if (++rx == XN) { rx = 0; ++ry; }
+        }
+    };
+    Ops cur, nxt;
+    load(cur);
+    for (long v0 = c0; v0 < c1; v0 += 32) {
+        if (v0 + 32 < c1) load(nxt);
+#pragma unroll
+        for (int nb = 0; nb < 2; ++nb) {
+            const u32x4 blk = cur.blk[nb];
             // dz = 0
             const bf16x8 B0 = as_bf(blk);
             // dz = -1: elements e-1 .. e+6 (element 0 invalid at z0 == 0)
             u32x4 lo;
-            lo[0] = __builtin_amdgcn_alignbyte(blk[0], prev, 2);
+            lo[0] = __builtin_amdgcn_alignbyte(blk[0], cur.prev[nb], 2);
             lo[1] = __builtin_amdgcn_alignbyte(blk[1], blk[0], 2);
             lo[2] = __builtin_amdgcn_alignbyte(blk[2], blk[1], 2);
             lo[3] = __builtin_amdgcn_alignbyte(blk[3], blk[2], 2);
-            if (z0 == 0) lo[0] &= 0xffff0000u;
+            if (cur.z0 == 0) lo[0] &= 0xffff0000u;
             // dz = +1: elements e+1 .. e+8 (element 7 invalid at z0 + 8 == ZN)
             u32x4 hi;
             hi[0] = __builtin_amdgcn_alignbyte(blk[1], blk[0], 2);
             hi[1] = __builtin_amdgcn_alignbyte(blk[2], blk[1], 2);
             hi[2] = __builtin_amdgcn_alignbyte(blk[3], blk[2], 2);
-            hi[3] = __builtin_amdgcn_alignbyte(next, blk[3], 2);
-            if (z0 + 8 == ZN) hi[3] &= 0x0000ffffu;
+            hi[3] = __builtin_amdgcn_alignbyte(cur.next[nb], blk[3], 2);
+            if (cur.z0 + 8 == ZN) hi[3] &= 0x0000ffffu;
             const bf16x8 Bm = as_bf(lo), Bp = as_bf(hi);
-            acc[0][0][nb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A0, Bm, acc[0][0][nb], 0, 0, 0);
-            acc[0][1][nb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A1, Bm, acc[0][1][nb], 0, 0, 0);
-            acc[1][0][nb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A0, B0, acc[1][0][nb], 0, 0, 0);
-            acc[1][1][nb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A1, B0, acc[1][1][nb], 0, 0, 0);
-            acc[2][0][nb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A0, Bp, acc[2][0][nb], 0, 0, 0);
-            acc[2][1][nb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A1, Bp, acc[2][1][nb], 0, 0, 0);
+            acc[0][0][nb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(cur.A0, Bm, acc[0][0][nb], 0, 0, 0);
+            acc[0][1][nb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(cur.A1, Bm, acc[0][1][nb], 0, 0, 0);
+            acc[1][0][nb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(cur.A0, B0, acc[1][0][nb], 0, 0, 0);
+            acc[1][1][nb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(cur.A1, B0, acc[1][1][nb], 0, 0, 0);
+            acc[2][0][nb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(cur.A0, Bp, acc[2][0][nb], 0, 0, 0);
+            acc[2][1][nb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(cur.A1, Bp, acc[2][1][nb], 0, 0, 0);
         }
+        cur = nxt;
     }
     // D: rows co = mb*16 + 4*kq + i, column ci = nb*16 + m
     float* P = part + (long)blockIdx.x * 27 * CH * CH;
@@ -506,11 +541,26 @@ k_wgrad_first(const __bf16* __restrict__ g_cl, const float* __restrict__ x, floa
     int z = (int)(c0 % ZN);
     const long xy = c0 / ZN;
     int xx = (int)(xy % XN), y = (int)(xy / XN);
-    for (long v = c0; v < c1; ++v) {
+    // 4 voxels per step (chunk and ZN are multiples of 8: the 4 share a column);
+    // loads are unconditional at a valid index and their values selected, so
+    // all 32 are in flight together
+    for (long v = c0; v < c1; v += 4) {
+        float g[4][4], xa[4][4];
 #pragma unroll
-        for (int k = 0; k < 4; ++k)
-            if (tap_in(q[k], y, xx, z, YN, XN, ZN)) acc[k] += bf(g_cl[v * CH + q[k].c]) * x[v + q[k].off];
-        if (++z == ZN) {
+        for (int i = 0; i < 4; ++i)
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                const bool ok = tap_in(q[k], y, xx, z + i, YN, XN, ZN);
+                g[i][k] = bf(g_cl[(v + i) * CH + q[k].c]);
+                xa[i][k] = x[ok ? v + i + q[k].off : v + i];
+                xa[i][k] = ok ? xa[i][k] : 0.0f;
+            }
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+#pragma unroll
+            for (int k = 0; k < 4; ++k) acc[k] += g[i][k] * xa[i][k];
+        z += 4;
+        if (z == ZN) {
             z = 0;
             if (++xx == XN) { xx = 0; ++y; }
         }
@@ -534,12 +584,25 @@ k_wgrad_last(const float* __restrict__ g5, const __bf16* __restrict__ a_cl, floa
     int z = (int)(c0 % ZN);
     const long xy = c0 / ZN;
     int xx = (int)(xy % XN), y = (int)(xy / XN);
-    for (long v = c0; v < c1; ++v) {
-        const float gv = g5[v];
+    // 4 voxels per step, loads unconditional at a valid index (as in k_wgrad_first)
+    for (long v = c0; v < c1; v += 4) {
+        float gv[4], aa[4][4];
 #pragma unroll
-        for (int k = 0; k < 4; ++k)
-            if (tap_in(q[k], y, xx, z, YN, XN, ZN)) acc[k] += gv * bf(a_cl[(v + q[k].off) * CH + q[k].c]);
-        if (++z == ZN) {
+        for (int i = 0; i < 4; ++i) {
+            gv[i] = g5[v + i];
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                const bool ok = tap_in(q[k], y, xx, z + i, YN, XN, ZN);
+                aa[i][k] = bf(a_cl[(ok ? v + i + q[k].off : v + i) * CH + q[k].c]);
+                aa[i][k] = ok ? aa[i][k] : 0.0f;
+            }
+        }
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+#pragma unroll
+            for (int k = 0; k < 4; ++k) acc[k] += gv[i] * aa[i][k];
+        z += 4;
+        if (z == ZN) {
             z = 0;
             if (++xx == XN) { xx = 0; ++y; }
         }
