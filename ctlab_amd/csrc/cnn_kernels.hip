@@ -54,8 +54,13 @@ __device__ __forceinline__ float bf(const __bf16 v) { return (float)v; }
 // channels, zero outside the volume) and the plane's 9 taps of weights are
 // staged in LDS (21.8 + 18.4 KB: 4 blocks per CU); the 9 taps then read their
 // MFMA operands from LDS, so each input voxel is fetched from L2 about 1.3
-// times per plane instead of 9.  Registers are held to 128 (4 wavefronts per
-// SIMD, what the LDS allows): 136 registers and 3 wavefronts ran 14 % slower.  Tiles are dealt to the 8 XCDs in contiguous
+// times per plane instead of 9.  The next plane's brick is loaded into
+// registers while the current plane's MFMAs run (a plane's 72 MFMAs per
+// wavefront are shorter than one round of global loads; fetching after the
+// barrier left the MFMAs idle: 1.52 -> 1.16 ms per layer at 512x512x64).
+// Prefetching the weights as well needs 168 registers and ran slower.
+// Registers are held to 128 (4 wavefronts per SIMD, what the LDS allows):
+// 136 registers and 3 wavefronts ran 14 % slower.  Tiles are dealt to the 8 XCDs in contiguous
 // ranges (y rows of one range share their input rows in that XCD's L2).
 // ---------------------------------------------------------------------------
 constexpr int kBX = kTX + 2, kBZ = kTZ + 2;         // input brick of one plane
@@ -95,21 +100,33 @@ k_conv32(const __bf16* __restrict__ in, const __bf16* __restrict__ w, const floa
     for (int b = 0; b < 4; ++b) acc[b][0] = acc[b][1] = (f32x4){0.f, 0.f, 0.f, 0.f};
 
     const bf16x8 zero8 = (bf16x8){0, 0, 0, 0, 0, 0, 0, 0};
+    constexpr int kNA = (kBrick * CH / 8 + 255) / 256;
+    bf16x8 pa[kNA];
+    auto fetch = [&](int plane) {
+        const int yy = y + plane - 1;
+        const bool row_in = (unsigned)yy < (unsigned)YN;
+#pragma unroll
+        for (int j = 0; j < kNA; ++j) {
+            const int i = threadIdx.x + 256 * j;
+            const int p = i >> 2, q = i & 3;
+            const int gx = x0 - 1 + p / kBZ, gz = z0 - 1 + p % kBZ;
+            pa[j] = zero8;
+            if (i < kBrick * CH / 8 && row_in && (unsigned)gx < (unsigned)XN && (unsigned)gz < (unsigned)ZN)
+                pa[j] = ((const bf16x8*)in)[(((long)yy * XN + gx) * ZN + gz) * (CH / 8) + q];
+        }
+    };
+    fetch(0);
     for (int plane = 0; plane < 3; ++plane) {
         __syncthreads();   // the previous plane's operands are no longer read
         for (int i = threadIdx.x; i < 9 * CH * CH / 8; i += 256)
             sW[i] = ((const bf16x8*)w)[plane * 9 * CH * CH / 8 + i];
-        const int yy = y + plane - 1;
-        const bool row_in = (unsigned)yy < (unsigned)YN;
-        for (int i = threadIdx.x; i < kBrick * CH / 8; i += 256) {
-            const int p = i >> 2, q = i & 3;
-            const int gx = x0 - 1 + p / kBZ, gz = z0 - 1 + p % kBZ;
-            bf16x8 v = zero8;
-            if (row_in && (unsigned)gx < (unsigned)XN && (unsigned)gz < (unsigned)ZN)
-                v = ((const bf16x8*)in)[(((long)yy * XN + gx) * ZN + gz) * (CH / 8) + q];
-            sA[i] = v;
+#pragma unroll
+        for (int j = 0; j < kNA; ++j) {
+            const int i = threadIdx.x + 256 * j;
+            if (i < kBrick * CH / 8) sA[i] = pa[j];
         }
         __syncthreads();
+        if (plane < 2) fetch(plane + 1);   // in flight during this plane's MFMAs
 #pragma unroll
         for (int tt = 0; tt < 9; ++tt) {
             const int shift = (tt / 3 - 1) * kBZ + (tt % 3 - 1);      // (dx, dz) in the brick
