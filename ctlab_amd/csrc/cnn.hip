@@ -39,7 +39,7 @@ constexpr long kOffW[5] = {0, 27 * CH + CH, 27 * CH + CH + kW32 + CH, 27 * CH + 
 constexpr long kLenW[5] = {27 * CH, kW32, kW32, kW32, 27 * CH};
 constexpr long kLenB[5] = {CH, CH, CH, CH, 1};
 constexpr long kNParams = 27 * CH + CH + 3 * (kW32 + CH) + 27 * CH + 1;
-constexpr long kWgradChunk = 8192;     // voxels per wgrad partial (multiple of 32)
+constexpr long kWgradChunk = 8192;     // voxels per wgrad partial (multiple of 128: 4 wavefronts x runs of 32)
 constexpr long kVecChunk = 512;        // voxels per VALU wgrad partial
 constexpr long kRedTmp = 2048 * 64;    // reduce_chunks: groups x outputs <= (2048 / ceil(n/64)) * n
 }  // namespace
@@ -153,7 +153,7 @@ int backward(ctlab_cnn* N, const float* x, hipStream_t s)
     int cur = 0;   // g_*[cur] holds dL/dz of layer l+1 (l = 3, 2, 1 below: layers 4, 3, 2)
     for (int l = 3; l >= 1; --l) {
         // weight / bias gradients of layer l+1 (index l): input a_{l} = a_cl[l-1]
-        hipLaunchKernelGGL(k_wgrad32, dim3((unsigned)N->nchunks, 9), dim3(64), 0, s, N->g_cm[cur], N->a_cm[l - 1],
+        hipLaunchKernelGGL(k_wgrad32, dim3((unsigned)N->nchunks, 9), dim3(256), 0, s, N->g_cm[cur], N->a_cm[l - 1],
                            N->part, XN, YN, ZN, kWgradChunk);
         reduce_chunks(N, N->part, (int)N->nchunks, (int)kW32, G + kOffW[l], s);
         hipLaunchKernelGGL(k_bias_grad_partial, dim3((unsigned)N->nvchunks), dim3(256), 0, s, N->g_cm[cur], n,

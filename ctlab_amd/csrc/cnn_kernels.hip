@@ -390,28 +390,33 @@ k_dgrad_last(const float* __restrict__ g5, const float* __restrict__ w5, const _
 // ---------------------------------------------------------------------------
 // weight gradient of a 32 -> 32 layer on MFMA, K = voxels:
 //   dW[t][co][ci] = sum_v g[v][co] * a[v + off(t)][ci]
-// from channel-major g_cm, a_cm.  One wavefront = one (dy, dx) pair (3 taps,
-// dz = -1, 0, +1) over a chunk of voxels; needs ZN % 8 == 0 so that a run of
-// 8 voxels stays in one z-column.  The dz = +-1 runs are the aligned run
-// shifted by one element, assembled with alignbyte from a 16-byte load and one
-// neighbouring dword.  Partial sums per chunk: part[chunk][27][32][32].
+// from channel-major g_cm, a_cm.  One block = one (dy, dx) pair (3 taps,
+// dz = -1, 0, +1) over a chunk of voxels, each of its 4 wavefronts a quarter
+// of the chunk (their sums added in LDS in a fixed order); needs ZN % 8 == 0
+// so that a run of 8 voxels stays in one z-column.  The dz = +-1 runs are the
+// aligned run shifted by one element, assembled with alignbyte from a 16-byte
+// load and one neighbouring dword.  Partial sums per chunk:
+// part[chunk][27][32][32].
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ bf16x8 as_bf(u32x4 u)
 {
     return __builtin_bit_cast(bf16x8, u);
 }
 
-__global__ void __launch_bounds__(64)
+__global__ void __launch_bounds__(256)
 k_wgrad32(const __bf16* __restrict__ g_cm, const __bf16* __restrict__ a_cm, float* __restrict__ part,
           int XN, int YN, int ZN, long chunk)
 {
+    __shared__ float sR[3][48][64];               // 36,864 B: wavefronts 1..3's sums
     const long nvox = (long)XN * YN * ZN;
-    const int lane = threadIdx.x;
+    const int lane = threadIdx.x & 63;
+    const int wv = threadIdx.x >> 6;
     const int m = lane & 15, kq = lane >> 4;
     const int pair = blockIdx.y;                   // 0..8 = (dy+1)*3 + (dx+1)
     const int dy = pair / 3 - 1, dx = pair % 3 - 1;
-    const long c0 = blockIdx.x * chunk;
-    const long c1 = min(c0 + chunk, nvox);
+    const long sub = chunk / 4;                    // chunk % 128 == 0
+    const long c0 = blockIdx.x * chunk + wv * sub;
+    const long c1 = min(c0 + sub, nvox);
     f32x4 acc[3][2][2];
 #pragma unroll
     for (int d = 0; d < 3; ++d)
@@ -501,6 +506,29 @@ k_wgrad32(const __bf16* __restrict__ g_cm, const __bf16* __restrict__ a_cm, floa
         }
         cur = nxt;
     }
+    // wavefronts 1..3 hand their sums to wavefront 0, which adds them in order
+    if (wv > 0) {
+#pragma unroll
+        for (int d = 0; d < 3; ++d)
+#pragma unroll
+            for (int mb = 0; mb < 2; ++mb)
+#pragma unroll
+                for (int nb = 0; nb < 2; ++nb)
+#pragma unroll
+                    for (int i = 0; i < 4; ++i) sR[wv - 1][((d * 2 + mb) * 2 + nb) * 4 + i][lane] = acc[d][mb][nb][i];
+    }
+    __syncthreads();
+    if (wv > 0) return;
+#pragma unroll
+    for (int w = 0; w < 3; ++w)
+#pragma unroll
+        for (int d = 0; d < 3; ++d)
+#pragma unroll
+            for (int mb = 0; mb < 2; ++mb)
+#pragma unroll
+                for (int nb = 0; nb < 2; ++nb)
+#pragma unroll
+                    for (int i = 0; i < 4; ++i) acc[d][mb][nb][i] += sR[w][((d * 2 + mb) * 2 + nb) * 4 + i][lane];
     // D: rows co = mb*16 + 4*kq + i, column ci = nb*16 + m
     float* P = part + (long)blockIdx.x * 27 * CH * CH;
 #pragma unroll
@@ -516,75 +544,89 @@ k_wgrad32(const __bf16* __restrict__ g_cm, const __bf16* __restrict__ a_cm, floa
     }
 }
 
-// Thread (t, c) pair of a first / last layer weight gradient: tap offset and
-// displacement, fixed over the voxel loop (threads past 27*32 stay idle).
-struct TapPair {
-    int off, dy, dx, dz, c;
+// Weight gradients of the thin layers (VALU), one block per chunk of voxels,
+// partials part[chunk][27*32].  Thread r = tid % 128 < 108 owns tap t = r / 4
+// and the 8 channels 8*(r % 4)..+7 of it: per voxel one 16-byte load of those
+// channels and one scalar feed 8 FMAs.  The block's two halves (tid / 128)
+// take the two halves of the chunk, 4 voxels per step (chunk and ZN are
+// multiples of 8, so the 4 share a z-column; coordinates are stepped), and
+// their sums are added in LDS in a fixed order.  Loads outside the volume go
+// to a valid index and their value is replaced by 0, so none waits on a branch.
+struct ThinTap {
+    int t, o, dy, dx, dz;
+    long off;
     bool live;
 };
-__device__ __forceinline__ TapPair tap_pair(int idx, int XN, int ZN)
+__device__ __forceinline__ ThinTap thin_tap(int XN, int ZN)
 {
-    TapPair q;
-    q.live = idx < 27 * CH;
-    const int t = q.live ? idx / CH : 0;
-    q.c = idx % CH;
-    q.off = tap_off(t, XN, ZN);
-    q.dy = t / 9 - 1;
-    q.dx = (t / 3) % 3 - 1;
-    q.dz = t % 3 - 1;
+    ThinTap q;
+    const int r = threadIdx.x & 127;
+    q.live = r < 27 * 4;
+    q.t = q.live ? r >> 2 : 0;
+    q.o = r & 3;
+    q.dy = q.t / 9 - 1;
+    q.dx = (q.t / 3) % 3 - 1;
+    q.dz = q.t % 3 - 1;
+    q.off = tap_off(q.t, XN, ZN);
     return q;
 }
-__device__ __forceinline__ bool tap_in(const TapPair& q, int y, int x, int z, int YN, int XN, int ZN)
+
+// the two halves' sums -> part[blockIdx.x][t*32 + 8*o + e]
+__device__ __forceinline__ void thin_store(const ThinTap& q, float (&acc)[8], float (*sR)[8], float* part)
 {
-    return q.live && (unsigned)(y + q.dy) < (unsigned)YN && (unsigned)(x + q.dx) < (unsigned)XN &&
-           (unsigned)(z + q.dz) < (unsigned)ZN;
+    const int r = threadIdx.x & 127;
+    if (threadIdx.x >= 128) {
+#pragma unroll
+        for (int e = 0; e < 8; ++e) sR[r][e] = acc[e];
+    }
+    __syncthreads();
+    if (threadIdx.x < 128 && q.live) {
+#pragma unroll
+        for (int e = 0; e < 8; ++e) part[(long)blockIdx.x * 27 * CH + q.t * CH + 8 * q.o + e] = acc[e] + sR[r][e];
+    }
 }
 
 // weight gradient of the first layer: dW1[t][co] = sum_v g1[v][co] * x[v + off(t)]
-// one block per chunk; thread (t, co) pairs; partials part[chunk][27*32].  The
-// gradient is read channel-last (g_cl[v][co]): the 64 lanes of a wavefront,
-// two taps x 32 channels, then read one 64-byte line per voxel.  The voxel
-// coordinates step along with v (z fastest), without divisions in the loop.
+// (g1 channels-last)
 __global__ void __launch_bounds__(256)
 k_wgrad_first(const __bf16* __restrict__ g_cl, const float* __restrict__ x, float* __restrict__ part,
               int XN, int YN, int ZN, long chunk)
 {
+    __shared__ float sR[128][8];
     const long nvox = (long)XN * YN * ZN;
-    const long c0 = blockIdx.x * chunk, c1 = min(c0 + chunk, nvox);
-    TapPair q[4];
-#pragma unroll
-    for (int k = 0; k < 4; ++k) q[k] = tap_pair(threadIdx.x + 256 * k, XN, ZN);
-    float acc[4] = {0.f, 0.f, 0.f, 0.f};
-    int z = (int)(c0 % ZN);
-    const long xy = c0 / ZN;
-    int xx = (int)(xy % XN), y = (int)(xy / XN);
-    // 4 voxels per step (chunk and ZN are multiples of 8: the 4 share a column);
-    // loads are unconditional at a valid index and their values selected, so
-    // all 32 are in flight together
+    const ThinTap q = thin_tap(XN, ZN);
+    const long half = chunk / 2;
+    const long c0 = blockIdx.x * chunk + (threadIdx.x >> 7) * half, c1 = min(c0 + half, nvox);
+    float acc[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+    int z = 0, xx = 0, y = 0;
+    if (c0 < nvox) {
+        z = (int)(c0 % ZN);
+        const long xy = c0 / ZN;
+        xx = (int)(xy % XN);
+        y = (int)(xy / XN);
+    }
     for (long v = c0; v < c1; v += 4) {
-        float g[4][4], xa[4][4];
+        const bool col = q.live && (unsigned)(y + q.dy) < (unsigned)YN && (unsigned)(xx + q.dx) < (unsigned)XN;
+        bf16x8 g[4];
+        float xa[4];
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const bool ok = col && (unsigned)(z + i + q.dz) < (unsigned)ZN;
+            g[i] = *(const bf16x8*)(g_cl + (v + i) * CH + 8 * q.o);
+            const float xv = x[ok ? v + i + q.off : v + i];
+            xa[i] = ok ? xv : 0.0f;
+        }
 #pragma unroll
         for (int i = 0; i < 4; ++i)
 #pragma unroll
-            for (int k = 0; k < 4; ++k) {
-                const bool ok = tap_in(q[k], y, xx, z + i, YN, XN, ZN);
-                g[i][k] = bf(g_cl[(v + i) * CH + q[k].c]);
-                xa[i][k] = x[ok ? v + i + q[k].off : v + i];
-                xa[i][k] = ok ? xa[i][k] : 0.0f;
-            }
-#pragma unroll
-        for (int i = 0; i < 4; ++i)
-#pragma unroll
-            for (int k = 0; k < 4; ++k) acc[k] += g[i][k] * xa[i][k];
+            for (int e = 0; e < 8; ++e) acc[e] += bf(g[i][e]) * xa[i];
         z += 4;
         if (z == ZN) {
             z = 0;
             if (++xx == XN) { xx = 0; ++y; }
         }
     }
-#pragma unroll
-    for (int k = 0; k < 4; ++k)
-        if (q[k].live) part[(long)blockIdx.x * 27 * CH + threadIdx.x + 256 * k] = acc[k];
+    thin_store(q, acc, sR, part);
 }
 
 // weight gradient of the last layer: dW5[t][ci] = sum_v g5[v] * a4[v + off(t)][ci]
@@ -592,41 +634,43 @@ __global__ void __launch_bounds__(256)
 k_wgrad_last(const float* __restrict__ g5, const __bf16* __restrict__ a_cl, float* __restrict__ part,
              int XN, int YN, int ZN, long chunk)
 {
+    __shared__ float sR[128][8];
     const long nvox = (long)XN * YN * ZN;
-    const long c0 = blockIdx.x * chunk, c1 = min(c0 + chunk, nvox);
-    TapPair q[4];
-#pragma unroll
-    for (int k = 0; k < 4; ++k) q[k] = tap_pair(threadIdx.x + 256 * k, XN, ZN);
-    float acc[4] = {0.f, 0.f, 0.f, 0.f};
-    int z = (int)(c0 % ZN);
-    const long xy = c0 / ZN;
-    int xx = (int)(xy % XN), y = (int)(xy / XN);
-    // 4 voxels per step, loads unconditional at a valid index (as in k_wgrad_first)
+    const ThinTap q = thin_tap(XN, ZN);
+    const long half = chunk / 2;
+    const long c0 = blockIdx.x * chunk + (threadIdx.x >> 7) * half, c1 = min(c0 + half, nvox);
+    float acc[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+    int z = 0, xx = 0, y = 0;
+    if (c0 < nvox) {
+        z = (int)(c0 % ZN);
+        const long xy = c0 / ZN;
+        xx = (int)(xy % XN);
+        y = (int)(xy / XN);
+    }
     for (long v = c0; v < c1; v += 4) {
-        float gv[4], aa[4][4];
+        const bool col = q.live && (unsigned)(y + q.dy) < (unsigned)YN && (unsigned)(xx + q.dx) < (unsigned)XN;
+        bf16x8 a[4];
+        float gv[4];
+        bool ok[4];
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
+            ok[i] = col && (unsigned)(z + i + q.dz) < (unsigned)ZN;
             gv[i] = g5[v + i];
-#pragma unroll
-            for (int k = 0; k < 4; ++k) {
-                const bool ok = tap_in(q[k], y, xx, z + i, YN, XN, ZN);
-                aa[i][k] = bf(a_cl[(ok ? v + i + q[k].off : v + i) * CH + q[k].c]);
-                aa[i][k] = ok ? aa[i][k] : 0.0f;
-            }
+            a[i] = *(const bf16x8*)(a_cl + (ok[i] ? v + i + q.off : v + i) * CH + 8 * q.o);
         }
 #pragma unroll
-        for (int i = 0; i < 4; ++i)
+        for (int i = 0; i < 4; ++i) {
+            const float gi = ok[i] ? gv[i] : 0.0f;
 #pragma unroll
-            for (int k = 0; k < 4; ++k) acc[k] += gv[i] * aa[i][k];
+            for (int e = 0; e < 8; ++e) acc[e] += gi * bf(a[i][e]);
+        }
         z += 4;
         if (z == ZN) {
             z = 0;
             if (++xx == XN) { xx = 0; ++y; }
         }
     }
-#pragma unroll
-    for (int k = 0; k < 4; ++k)
-        if (q[k].live) part[(long)blockIdx.x * 27 * CH + threadIdx.x + 256 * k] = acc[k];
+    thin_store(q, acc, sR, part);
 }
 
 // bias-gradient partials: part[j][c] = sum over chunk j of g_cm[c][v] (block j;
