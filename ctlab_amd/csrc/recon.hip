@@ -38,8 +38,8 @@ hipError_t rk_beta(const float* r0, const float* r1, const float* d, int cases, 
                    double* part, double* out, hipStream_t s);
 hipError_t rk_dot(const float* x, const float* y, long n, double* part, double* out, hipStream_t s);
 int red_scratch_doubles();
-hipError_t rk_tv_prox(const float* b, float* x, float lam, int iters, int XN, int YN, int ZN, float* p,
-                      float* r, float* tmp, hipStream_t s);
+hipError_t rk_tv_prox(const float* b, float* x, float lam, int iters, int XN, int YN, int ZN, float* work,
+                      hipStream_t s);
 }  // namespace ctlab
 
 using namespace ctlab;
@@ -399,8 +399,7 @@ int ctlab_tv_prox(const float* d_b, float* d_x, float lam, int iters, int XN, in
         return CTLAB_ERR_ARG;
     }
     const long n = (long)XN * YN * ZN;
-    if (chk(rk_tv_prox(d_b, d_x, lam, iters, XN, YN, ZN, d_work, d_work + 3 * n, d_work + 6 * n,
-                       (hipStream_t)stream), "tv prox"))
+    if (chk(rk_tv_prox(d_b, d_x, lam, iters, XN, YN, ZN, d_work, (hipStream_t)stream), "tv prox"))
         return ctlab_last_error();
     return CTLAB_OK;
 }
@@ -420,7 +419,7 @@ int ctlab_fista_tv(ctlab_recon* R, float* d_x, int iters, float tv_lambda, int t
         L = (float)ctlab_recon_lipschitz(R, 20, stream) * 1.01f;
         if (!(L > 0)) return ctlab_last_error() ? ctlab_last_error() : CTLAB_ERR_GEOMETRY;
     }
-    if (!R->d_tv && chk(hipMalloc(&R->d_tv, 7 * R->nvox * sizeof(float)), "hipMalloc")) return ctlab_last_error();
+    if (!R->d_tv && chk(hipMalloc(&R->d_tv, 9 * R->nvox * sizeof(float)), "hipMalloc")) return ctlab_last_error();
     const long n = R->nvox;
     float* y = R->d_vol2;      // extrapolated point
     float* xold = R->d_vol3;
@@ -433,8 +432,7 @@ int ctlab_fista_tv(ctlab_recon* R, float* d_x, int iters, float tv_lambda, int t
         if (std::isnan(f)) return ctlab_last_error();
         if (h_obj) h_obj[it] = f;
         if (chk(rk_xpaymz(z, y, -1.0f / L, R->d_vol, nullptr, n, s), "gradient step")) return ctlab_last_error();
-        if (chk(rk_tv_prox(z, d_x, tv_lambda / L, tv_iters, R->XN, R->YN, R->ZN, R->d_tv, R->d_tv + 3 * n,
-                           R->d_tv + 6 * n, s), "tv prox"))
+        if (chk(rk_tv_prox(z, d_x, tv_lambda / L, tv_iters, R->XN, R->YN, R->ZN, R->d_tv, s), "tv prox"))
             return ctlab_last_error();
         const double tn = 0.5 * (1.0 + std::sqrt(1.0 + 4.0 * t * t));
         // optional non-negativity: FISTA_functor with t1 = 1 is a pure clamp

@@ -24,7 +24,7 @@ struct ctlab_recon {
     float* d_vol2 = nullptr;
     float* d_vol3 = nullptr;
     float* d_vol4 = nullptr;
-    float* d_tv = nullptr;              // 7 * nvox for the FGP prox
+    float* d_tv = nullptr;              // 9 * nvox for the FGP prox
     double* d_part = nullptr;           // reduction partials
     double* d_red = nullptr;            // reduction results (4 doubles)
     bool have_data = false;

@@ -216,28 +216,45 @@ __global__ void __launch_bounds__(256) k_tv_primal(const float* __restrict__ b, 
     out[i] = b[i] - lam * v;
 }
 
-// p_new = proj_{|.|<=1}(r + step * L^T(tmp)); r = p_new + mom*(p_new - p_old); p = p_new
-__global__ void __launch_bounds__(256) k_tv_dual(const float* __restrict__ tmp, float* __restrict__ p,
-                                                 float* __restrict__ r, float step, float mom, int XN, int YN, int ZN,
-                                                 float rcpZN)
+// One FGP iteration, fused.  The extrapolated dual r = pc + mom*(pc - pp) (pc,
+// pp: the last two dual iterates, 3n floats each) is formed where it is read;
+// tmp = b - lam * L(r) is evaluated at the voxel and its three forward
+// neighbours (k_tv_primal's arithmetic); then
+//   pn = proj_{|.|<=1}(r + step * L^T(tmp))
+// goes to a third buffer, so no thread writes what another one reads.  Per
+// iteration this moves 9 floats per voxel through HBM (pc, pp, b, pn) where a
+// primal and a dual pass over stored r and tmp moved 17.
+__global__ void __launch_bounds__(256) k_tv_fgp(const float* __restrict__ b, const float* __restrict__ pc,
+                                                const float* __restrict__ pp, float mom, float lam, float step,
+                                                int XN, int YN, int ZN, float rcpZN, float* __restrict__ pn)
 {
     const long n = (long)XN * YN * ZN;
     const RowIdx q = row_index(XN, ZN, rcpZN);
     if (!q.ok) return;
     const long i = q.i;
-    const float t = tmp[i];
-    const float qx = q.x < XN - 1 ? t - tmp[i + ZN] : 0.0f;
-    const float qy = (int)blockIdx.y < YN - 1 ? t - tmp[i + (long)XN * ZN] : 0.0f;
-    const float qz = q.z < ZN - 1 ? t - tmp[i + 1] : 0.0f;
-    const float a0 = r[i] + step * qx, a1 = r[n + i] + step * qy, a2 = r[2 * n + i] + step * qz;
+    const int y = blockIdx.y;
+    const long ex = ZN, ey = (long)XN * ZN;
+    auto r = [&](int c, long j) {
+        const float a = pc[c * n + j];
+        return a + mom * (a - pp[c * n + j]);
+    };
+    auto tmp_at = [&](long j, int xj, int yj, int zj) {
+        float v = r(0, j) + r(1, j) + r(2, j);
+        if (xj > 0) v -= r(0, j - ex);
+        if (yj > 0) v -= r(1, j - ey);
+        if (zj > 0) v -= r(2, j - 1);
+        return b[j] - lam * v;
+    };
+    const float t = tmp_at(i, q.x, y, q.z);
+    const float qx = q.x < XN - 1 ? t - tmp_at(i + ex, q.x + 1, y, q.z) : 0.0f;
+    const float qy = y < YN - 1 ? t - tmp_at(i + ey, q.x, y + 1, q.z) : 0.0f;
+    const float qz = q.z < ZN - 1 ? t - tmp_at(i + 1, q.x, y, q.z + 1) : 0.0f;
+    const float a0 = r(0, i) + step * qx, a1 = r(1, i) + step * qy, a2 = r(2, i) + step * qz;
     const float nrm = sqrtf(a0 * a0 + a1 * a1 + a2 * a2);
-    const float s = nrm > 1.0f ? 1.0f / nrm : 1.0f;
-    const float p0 = a0 * s, p1 = a1 * s, p2 = a2 * s;
-    const float o0 = p[i], o1 = p[n + i], o2 = p[2 * n + i];
-    p[i] = p0; p[n + i] = p1; p[2 * n + i] = p2;
-    r[i] = p0 + mom * (p0 - o0);
-    r[n + i] = p1 + mom * (p1 - o1);
-    r[2 * n + i] = p2 + mom * (p2 - o2);
+    const float sc = nrm > 1.0f ? 1.0f / nrm : 1.0f;
+    pn[i] = a0 * sc;
+    pn[n + i] = a1 * sc;
+    pn[2 * n + i] = a2 * sc;
 }
 
 // ---------------------------------------------------------------------------
@@ -314,10 +331,11 @@ hipError_t rk_dot(const float* x, const float* y, long n, double* part, double* 
 
 int red_scratch_doubles() { return 2 * kRedBlocks; }
 
-// FGP prox: x = argmin 0.5||x-b||^2 + lam TV(x); p, r: 3n floats each, tmp: n floats.
-// Step 1/(4*ndim*lam); momentum from the FISTA t-sequence (host computed).
-hipError_t rk_tv_prox(const float* b, float* x, float lam, int iters, int XN, int YN, int ZN, float* p,
-                      float* r, float* tmp, hipStream_t s)
+// FGP prox: x = argmin 0.5||x-b||^2 + lam TV(x); work: 9n floats (three dual
+// iterates of 3n, rotated).  Step 1/(4*ndim*lam); momentum from the FISTA
+// t-sequence (host computed).
+hipError_t rk_tv_prox(const float* b, float* x, float lam, int iters, int XN, int YN, int ZN, float* work,
+                      hipStream_t s)
 {
     const long n = (long)XN * YN * ZN;
     if (lam <= 0.0f) {
@@ -328,17 +346,23 @@ hipError_t rk_tv_prox(const float* b, float* x, float lam, int iters, int XN, in
     const float rcp = 1.0f / (float)ZN;
     const int nd = (XN > 1) + (YN > 1) + (ZN > 1);
     const float step = 1.0f / (4.0f * (float)(nd > 0 ? nd : 1) * lam);
-    hipLaunchKernelGGL(k_fill, dim3(ew_grid(3 * n)), dim3(kEwBlock), 0, s, p, 0.0f, 3 * n);
-    hipLaunchKernelGGL(k_fill, dim3(ew_grid(3 * n)), dim3(kEwBlock), 0, s, r, 0.0f, 3 * n);
+    float* P[3] = {work, work + 3 * n, work + 6 * n};
+    // p_0 = 0 and the iterate before it = 0: the first r is 0
+    hipLaunchKernelGGL(k_fill, dim3(ew_grid(6 * n)), dim3(kEwBlock), 0, s, work, 0.0f, 6 * n);
+    int cur = 0, prev = 1;
+    float mom = 0.0f;
     double t = 1.0;
     for (int it = 0; it < iters; ++it) {
-        hipLaunchKernelGGL(k_tv_primal, grid, dim3(256), 0, s, b, r, lam, XN, YN, ZN, rcp, tmp);
+        const int nxt = 3 - cur - prev;
+        hipLaunchKernelGGL(k_tv_fgp, grid, dim3(256), 0, s, b, P[cur], P[prev], mom, lam, step, XN, YN, ZN, rcp,
+                           P[nxt]);
         const double tn = 0.5 * (1.0 + __builtin_sqrt(1.0 + 4.0 * t * t));
-        const float mom = (float)((t - 1.0) / tn);
-        hipLaunchKernelGGL(k_tv_dual, grid, dim3(256), 0, s, tmp, p, r, step, mom, XN, YN, ZN, rcp);
+        mom = (float)((t - 1.0) / tn);                  // extrapolates from this iteration's output
         t = tn;
+        prev = cur;
+        cur = nxt;
     }
-    hipLaunchKernelGGL(k_tv_primal, grid, dim3(256), 0, s, b, p, lam, XN, YN, ZN, rcp, x);
+    hipLaunchKernelGGL(k_tv_primal, grid, dim3(256), 0, s, b, P[cur], lam, XN, YN, ZN, rcp, x);
     return hipGetLastError();
 }
 

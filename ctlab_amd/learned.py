@@ -302,7 +302,7 @@ def fista_tv_run(rc, L, b_shape_vol, iters, tv_lambda, tv_iters, truth, zsl, dev
     z = torch.empty_like(x)
     xc = torch.empty_like(x)
     xt = torch.empty_like(x) if learned is not None else None
-    work = torch.empty(7 * n, device=dev)
+    work = torch.empty(9 * n, device=dev)
     t = 1.0
     f_x = rc.data_fidelity(x)
     in_loop = learned is not None and feedback

@@ -184,7 +184,7 @@ def cg_beta(gk, gk1, d, cases, stream=None):
 
 
 def tv_prox_into(b, x, lam, iters, shape, work, stream=None):
-    """x = TV prox of b with a caller-owned work buffer of 7*numel floats."""
+    """x = TV prox of b with a caller-owned work buffer of 9*numel floats."""
     YN, XN, ZN = shape
     _lib.check(_lib.lib().ctlab_tv_prox(_p(b), _p(x), lam, iters, XN, YN, ZN, _p(work), _stream_handle(stream)))
     return x
@@ -194,7 +194,7 @@ def tv_prox(b, lam, iters, shape, stream=None):
     import torch
     YN, XN, ZN = shape
     x = torch.empty_like(b)
-    work = torch.empty(7 * b.numel(), dtype=torch.float32, device=b.device)
+    work = torch.empty(9 * b.numel(), dtype=torch.float32, device=b.device)
     _lib.check(_lib.lib().ctlab_tv_prox(_p(b), _p(x), lam, iters, XN, YN, ZN, _p(work), _stream_handle(stream)))
     return x
 

@@ -268,7 +268,7 @@ int ctlab_soft_threshold(float* d_x, float g, long long n, void* stream);
 int ctlab_axpby(float a, const float* d_x, float b, float* d_y, long long n, void* stream);
 int ctlab_cg_beta(const float* d_gk, const float* d_gk1, const float* d_d, int cases, long long n,
                   double* h_numden, void* stream);
-/* isotropic TV prox (FGP, iters steps) on a (Z,X,Y) volume; d_work: 7*XN*YN*ZN floats */
+/* isotropic TV prox (FGP, iters steps) on a (Z,X,Y) volume; d_work: 9*XN*YN*ZN floats */
 int ctlab_tv_prox(const float* d_b, float* d_x, float lam, int iters, int XN, int YN, int ZN, float* d_work,
                   void* stream);
 

@@ -63,12 +63,16 @@ def test_ossart_update_soft_threshold_and_beta(cuda):
         assert abs(b_gpu - b_ref) <= 1e-9 * abs(b_ref)     # FP64 accumulation of FP32 inputs
 
 
-def test_tv_prox_matches_oracle(cuda):
+@pytest.mark.parametrize("iters,shape", [(15, (20, 24, 12)), (0, (20, 24, 12)), (1, (20, 24, 12)),
+                                         (2, (7, 5, 3)), (4, (1, 9, 16)), (3, (6, 1, 1))])
+def test_tv_prox_matches_oracle(cuda, iters, shape):
+    """FGP iterates (the fused kernel rotates three dual buffers and forms the
+    extrapolated dual on the fly) against the FP64 oracle, including 0 / 1
+    iterations and volumes thin in one or two axes."""
     rng = np.random.default_rng(1)
-    shape = (20, 24, 12)   # (YN, XN, ZN)
-    b = rng.random(shape).astype(np.float32)
-    x = RC.tv_prox(T(b, cuda), 0.08, 15, shape).cpu().numpy()
-    ref = O.tv_prox(b.astype(np.float64), shape[1], shape[0], shape[2], 0.08, 15)
+    b = rng.random(shape).astype(np.float32)      # shape = (YN, XN, ZN)
+    x = RC.tv_prox(T(b, cuda), 0.08, iters, shape).cpu().numpy()
+    ref = O.tv_prox(b.astype(np.float64), shape[1], shape[0], shape[2], 0.08, iters)
     assert rel_l2(x, ref) < 1e-5
 
 
