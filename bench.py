@@ -214,7 +214,7 @@ def learned_bench(args):
     # one learned step only, after N plain iterations (what a caller who stops at N runs): the
     # output's time is N plain iterations plus that one step
     single = {}
-    for N in range(10, args.iters + 1, 10):
+    for N in range(5, args.iters + 1, 5):
         _, hN, _ = LE.fista_tv_run(re_, Le, ge.vol_shape, N, lam, 10, ph, ze, dev, learned=dca, learned_every=N,
                                    feedback=False)
         single[N] = {"t_s": hN[-1]["t"], "psnr": hN[-1]["psnr"], "fidelity": hN[-1]["fidelity"],
@@ -520,7 +520,7 @@ def main():
     ap.add_argument("--problems", type=int, default=32)
     ap.add_argument("--subsets", type=int, default=10)
     ap.add_argument("--samples", type=int, default=2000)
-    ap.add_argument("--epochs", type=int, default=16)
+    ap.add_argument("--epochs", type=int, default=24)
     ap.add_argument("--lr-schedule", default="cosine", choices=("cosine", "step"),
                     help="--learned: Adam step sizes (cosine decay to 0.05x, or the last epoch at 0.3x)")
     ap.add_argument("--iters", type=int, default=60)
