@@ -238,17 +238,29 @@ __global__ void __launch_bounds__(256) k_tv_fgp(const float* __restrict__ b, con
         const float a = pc[c * n + j];
         return a + mom * (a - pp[c * n + j]);
     };
-    auto tmp_at = [&](long j, int xj, int yj, int zj) {
+    // Every load is unconditional: a neighbour that does not exist is read at
+    // index i (valid) and its term masked by a select, so all 46 loads are in
+    // flight together (behind branches they came in four or five round trips).
+    // Subtracting an exact 0 leaves each sum bit for bit as k_tv_primal forms it.
+    const bool lx = q.x > 0, ly = y > 0, lz = q.z > 0;                     // backward neighbours
+    const bool hx = q.x < XN - 1, hy = y < YN - 1, hz = q.z < ZN - 1;      // forward neighbours
+    auto nb = [&](bool ok, long j) { return ok ? j : i; };
+    auto tmp_at = [&](long j, bool mx, long jx, bool my, long jy, bool mz, long jz) {
         float v = r(0, j) + r(1, j) + r(2, j);
-        if (xj > 0) v -= r(0, j - ex);
-        if (yj > 0) v -= r(1, j - ey);
-        if (zj > 0) v -= r(2, j - 1);
+        const float sx = r(0, jx), sy = r(1, jy), sz = r(2, jz);
+        v -= mx ? sx : 0.0f;
+        v -= my ? sy : 0.0f;
+        v -= mz ? sz : 0.0f;
         return b[j] - lam * v;
     };
-    const float t = tmp_at(i, q.x, y, q.z);
-    const float qx = q.x < XN - 1 ? t - tmp_at(i + ex, q.x + 1, y, q.z) : 0.0f;
-    const float qy = y < YN - 1 ? t - tmp_at(i + ey, q.x, y + 1, q.z) : 0.0f;
-    const float qz = q.z < ZN - 1 ? t - tmp_at(i + 1, q.x, y, q.z + 1) : 0.0f;
+    const float t = tmp_at(i, lx, nb(lx, i - ex), ly, nb(ly, i - ey), lz, nb(lz, i - 1));
+    const long jX = nb(hx, i + ex), jY = nb(hy, i + ey), jZ = nb(hz, i + 1);
+    const float tX = tmp_at(jX, true, i, ly, nb(hx && ly, i + ex - ey), lz, nb(hx && lz, i + ex - 1));
+    const float tY = tmp_at(jY, lx, nb(hy && lx, i + ey - ex), true, i, lz, nb(hy && lz, i + ey - 1));
+    const float tZ = tmp_at(jZ, lx, nb(hz && lx, i + 1 - ex), ly, nb(hz && ly, i + 1 - ey), true, i);
+    const float qx = hx ? t - tX : 0.0f;
+    const float qy = hy ? t - tY : 0.0f;
+    const float qz = hz ? t - tZ : 0.0f;
     const float a0 = r(0, i) + step * qx, a1 = r(1, i) + step * qy, a2 = r(2, i) + step * qz;
     const float nrm = sqrtf(a0 * a0 + a1 * a1 + a2 * a2);
     const float sc = nrm > 1.0f ? 1.0f / nrm : 1.0f;
