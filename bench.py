@@ -166,8 +166,8 @@ def learned_bench(args):
         params, losses, t_train, steps = LE.train(xs, ts, tshape, args.epochs, lr=1e-3, log=log,
                                                   schedule=args.lr_schedule)
         del xs, ts
-        np.save(os.path.join(ROOT, "gpurun_out", "learned_prox_params.npy"), params) if os.path.isdir(
-            os.path.join(ROOT, "gpurun_out")) else None
+        if args.save_params:
+            np.save(args.save_params, params)
     # validation: the trained step inside FISTA on the validation phantom
     net_v = LE.ScaledCNN(gv.vol_shape, params)
     _, hvl, _ = LE.fista_tv_run(rv, Lt, gv.vol_shape, args.iters, lam, 10, ph_v, zv, dev, learned=net_v)
@@ -525,6 +525,7 @@ def main():
                     help="--learned: Adam step sizes (cosine decay to 0.05x, or the last epoch at 0.3x)")
     ap.add_argument("--iters", type=int, default=60)
     ap.add_argument("--params", default="", help="--learned: trained parameters (.npy) instead of training")
+    ap.add_argument("--save-params", default="", help="--learned: write the trained parameters (.npy) here")
     ap.add_argument("--plain-sweep-iters", type=int, default=300,
                     help="--learned: iterations of the reported plain FISTA-TV weight sweep on the evaluation phantom")
     ap.add_argument("--dc-steps", type=int, default=3,

@@ -1,7 +1,7 @@
 // Learned proximal CNN kernels (gfx950): 5 layers of 3x3x3 convolution,
 // 1 -> 32 -> 32 -> 32 -> 32 -> 1, ReLU after layers 1-4, residual output
 // x + net(x).  BF16 weights and activations, FP32 accumulation.  No
-// counterpart in CTLAB (the north star adds it).
+// counterpart in CTLAB.
 //
 // Layouts (voxel v = (y*XN + x)*ZN + z, the volume's (Z,X,Y) order):
 //   channels-last  act_cl[v*32 + c]   (MFMA A-operand: 8 channels of a voxel = 16 B)

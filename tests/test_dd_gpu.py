@@ -1,6 +1,6 @@
 """Parity of the HIP projector / backprojector with the FP64 oracle.
 
-Bar (north star): FP32 projector within relative L2 <= 1e-5 of the FP64
+Bar: FP32 projector within relative L2 <= 1e-5 of the FP64
 oracle with FP64 integral images; the FP32-integral-image variant is reported
 and held to a looser bound.  Every call goes through the C ABI.
 """
