@@ -181,6 +181,19 @@ def test_fan_multislice_parity(cuda, sln):
     assert rel_l2(dd.DD2_multiGPU(g, prj, 2), O.back(g, prj)) <= TOL64
 
 
+def test_cfg1_fan_beam_parity(cuda):
+    """The 2-D fan-beam case at its full size (256x256 image of 10 seeded random
+    ellipses, 360 views, 888 arc cells of 1.0239 mm, SID 541, SDD 949) through
+    DD2_multiGPU: projection and backprojection against the FP64 oracle, which
+    equals the conventional overlap-merge DD at this size (test_oracle)."""
+    g = G.fan_multislice(XN=256, YN=256, SLN=1, dx=1.0, PN=360, DNU=888, col_size=1.0239, sid=541.0,
+                         sdd=949.0)
+    img = PH.voxelize(PH.random_ellipses(1, 10, 256, 256, g.dx), 256, 256, 1, g.dx, 1.0)
+    assert rel_l2(dd.DD2_multiGPU(g, img, 0), O.proj(g, img)) <= TOL64
+    prj = np.random.default_rng(3).random(g.nprj()).astype(np.float32)
+    assert rel_l2(dd.DD2_multiGPU(g, prj, 2), O.back(g, prj)) <= TOL64
+
+
 def test_host_abi_equals_plan(cuda):
     g = _cone(PN=20)
     vol = _phantom(g, 18)

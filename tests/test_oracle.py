@@ -131,6 +131,25 @@ def test_branchless_equals_conventional_2d():
     assert rel_l2(pf, pc) < 2e-6
 
 
+def test_branchless_equals_conventional_cfg1_size():
+    """The 2-D case at its full size (256x256 image of 10 seeded random
+    ellipses, 360 views over 360 degrees, 888 arc cells of 1.0239 mm, SID 541,
+    SDD 949): the branchless FP64 oracle equals the conventional overlap-merge
+    DD (the reference's DDM_*_proj / DDM_*_bproj restated) for both operators."""
+    g = G.fan_multislice(XN=256, YN=256, SLN=1, dx=1.0, PN=360, DNU=888, col_size=1.0239, sid=541.0,
+                         sdd=949.0)
+    img = PH.voxelize(PH.random_ellipses(1, 10, 256, 256, g.dx), 256, 256, 1, g.dx, 1.0).reshape(-1)
+    img = img.astype(np.float64)
+    assert img.max() > 0
+    bx, by, cx, cy = _fan_pts(g)
+    angs = g.angs.astype(np.float64)
+    pc = O.ddm_pts(False, 541.0, 1.0, 1.0, 256, 256, angs, bx, by, cx, cy, img)
+    assert rel_l2(O.proj(g, img), pc) < 1e-13
+    y = np.random.default_rng(3).random(g.nprj())
+    bc = O.ddm_pts(True, 541.0, 1.0, 1.0, 256, 256, angs, bx, by, cx, cy, y)
+    assert rel_l2(O.back(g, y), bc) < 1e-13
+
+
 def test_ddm_ea_api_matches_branchless_arc():
     """DDM_EA_proj's exact-arc boundaries equal DD3Boundaries of arc centres in
     angle (the arc is centred on the source), so only the two edge cells differ."""
