@@ -190,6 +190,11 @@ ctlab_cnn* ctlab_cnn_create(int XN, int YN, int ZN, int training, int gpunum)
         set_error(CTLAB_ERR_ARG, "training needs ZN %% 8 == 0 (got %d)", ZN);
         return nullptr;
     }
+    // one-dimensional launch grids of the tiled convolution kernels
+    if (conv32_tiles(XN, YN, ZN) > 0x7FFFFFF0L || conv_last_tiles(XN, YN, ZN) > 0x7FFFFFF0L) {
+        set_error(CTLAB_ERR_ARG, "volume %d x %d x %d exceeds the convolution kernels' grid", XN, YN, ZN);
+        return nullptr;
+    }
     ctlab_cnn* N = new ctlab_cnn();
     N->device = gpunum;
     N->XN = XN; N->YN = YN; N->ZN = ZN; N->training = training;

@@ -112,6 +112,11 @@ def test_plan_recon_batch_argument_checks():
     assert L.ctlab_cg_to_fidelity(None, None, 3, float("nan"), None, C.byref(done), None) == 1
     assert L.ctlab_cg_to_fidelity(None, None, 3, 0.0, None, C.byref(done), None) == 1
     assert _lib.last_error()[0] == 1
+    # CNN objects: sizes checked before any device call
+    assert not L.ctlab_cnn_create(0, 8, 8, 0, 0)
+    assert not L.ctlab_cnn_create(8, 8, 12, 1, 0)                 # training needs ZN % 8 == 0
+    assert not L.ctlab_cnn_create(1 << 20, 1 << 20, 1 << 10, 0, 0)  # beyond the kernels' grid
+    assert _lib.last_error()[0] == 1 and "grid" in _lib.last_error()[1]
 
 
 def test_multi_gpu_split_validation():
