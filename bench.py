@@ -45,21 +45,27 @@ def _traffic():
     return None
 
 
+def sample_views(PN, nviews):
+    """nviews views spread evenly over the rotation (every angle and both slab orientations)."""
+    return np.unique(np.linspace(0, PN, min(nviews, PN), endpoint=False).astype(np.int64))
+
+
 def cpu_baseline(g, vol, prj_gpu, back_fn, nviews):
-    """FP64 oracle on views [0, nviews): projection of the full volume, and the
+    """FP64 oracle on a sample of the views: projection of the full volume, and the
     backprojection of those views' GPU projections; also the GPU error per result."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O
-    sub = g.subset(np.arange(nviews))
+    views = sample_views(g.PN, nviews)
+    sub = g.subset(views)
     t0 = time.perf_counter()
     p_ref = O.proj(sub, vol)
     t1 = time.perf_counter()
-    y = prj_gpu[:nviews].reshape(-1).astype(np.float64)
+    y = prj_gpu[views].reshape(-1).astype(np.float64)
     b_ref = O.back(sub, y)
     t2 = time.perf_counter()
-    work = 2.0 * g.nvox() * nviews
-    p_gpu = prj_gpu[:nviews].reshape(-1).astype(np.float64)
-    b_gpu = back_fn(sub, prj_gpu[:nviews]).reshape(-1).astype(np.float64)
+    work = 2.0 * g.nvox() * views.size
+    p_gpu = prj_gpu[views].reshape(-1).astype(np.float64)
+    b_gpu = back_fn(sub, prj_gpu[views]).reshape(-1).astype(np.float64)
     par = errors(p_gpu, p_ref, b_gpu, b_ref)
     threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
     return {
@@ -67,9 +73,10 @@ def cpu_baseline(g, vol, prj_gpu, back_fn, nviews):
         "unit": "voxel·views/s",
         "cores": min(threads, os.cpu_count() or 1),
         "kind": "port",
-        "sample": f"FP64 oracle (OpenMP) proj + back of cfg2 geometry restricted to views 0..{nviews - 1}, "
-                  f"full 512x512x64 volume and 888x64 detector; proj {t1 - t0:.2f} s, back {t2 - t1:.2f} s",
-    }, dict(par, views_checked=nviews), (sub, p_ref, b_ref)
+        "sample": f"FP64 oracle (OpenMP) proj + back of the cfg2 geometry restricted to {views.size} views spread "
+                  f"evenly over the rotation, full 512x512x64 volume and 888x64 detector; proj {t1 - t0:.2f} s, "
+                  f"back {t2 - t1:.2f} s",
+    }, dict(par, views_checked=int(views.size)), (sub, p_ref, b_ref, views)
 
 
 def errors(p_gpu, p_ref, b_gpu, b_ref):
@@ -532,7 +539,8 @@ def main():
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--sat-bits", type=int, default=64)
-    ap.add_argument("--cpu-views", type=int, default=16)
+    ap.add_argument("--cpu-views", type=int, default=240,
+                    help="views (spread over the rotation) of the CPU baseline and parity sample")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-tune", action="store_true")
     ap.add_argument("--tiles", default="", help="fixed projector,backprojector tiles (e.g. 8,8) instead of tuning")
@@ -693,12 +701,11 @@ def main():
             r = pl.back(torch.from_numpy(np.ascontiguousarray(p_np)).to(dev))
             torch.cuda.synchronize()
             return r.cpu().numpy()
-        cb, par, (sub, p_ref, b_ref) = cpu_baseline(g, vol_np, prj.cpu().numpy(), back_fn, args.cpu_views)
+        cb, par, (sub, p_ref, b_ref, views) = cpu_baseline(g, vol_np, prj.cpu().numpy(), back_fn, args.cpu_views)
         out["cpu_baseline"] = cb
         out["parity"] = par
-        n = args.cpu_views
-        out["fp32_sat"]["parity"] = errors(prj32[:n].cpu().numpy().reshape(-1).astype(np.float64), p_ref,
-                                           back_fn(sub, prj.cpu().numpy()[:n], 32).reshape(-1).astype(np.float64),
+        out["fp32_sat"]["parity"] = errors(prj32.cpu().numpy()[views].reshape(-1).astype(np.float64), p_ref,
+                                           back_fn(sub, prj.cpu().numpy()[views], 32).reshape(-1).astype(np.float64),
                                            b_ref)
     print(json.dumps(out))
 
