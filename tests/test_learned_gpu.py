@@ -62,3 +62,32 @@ def test_trained_prox_gains_over_identity_and_safeguard_holds(cuda):
     for a, b in zip(hd, plain):
         if a["learned"]:
             assert a["fidelity"] <= b["fidelity"], (a, b)     # output mode: against the plain iterate
+
+    # adaptive stage: CG until the candidate's ||Ax - b||^2 is at the plain
+    # iterate's.  It stops at the first step that gets there (no step when the
+    # network's output already is), so the safeguard accepts its outputs, and
+    # the single-shot form (one learned step after N plain iterations) gives the
+    # plain iterate back with the same timing path when the step is rejected
+    dca = LE.DataConsistent(net, rc, to_fidelity=True, m_max=10)
+    f8 = rc.data_fidelity(x8)
+    out2 = torch.empty_like(x8)
+    dca(x8, out2, f_ref=f8)
+    f_ad = rc.data_fidelity(out2)
+    print(f"adaptive data consistency: {dca.last_steps} CG steps, ||Ax-b||^2 {f_ad:.6g} (plain iterate {f8:.6g})")
+    assert 0 <= dca.last_steps <= 10
+    if dca.last_steps < 10:
+        assert f_ad <= f8 * (1 + 1e-6)
+    if dca.last_steps > 0:
+        f_less = torch.empty_like(x8)
+        net(x8, f_less)
+        rc.cg(f_less, dca.last_steps - 1)
+        assert rc.data_fidelity(f_less) > f8 * (1 - 2e-5)     # one step fewer does not get there
+    _, ha, _ = LE.fista_tv_run(rc, L, g.vol_shape, 15, 0.3, 10, ph, None, cuda, learned=dca, learned_every=5,
+                               feedback=False)
+    for a, b in zip(ha, plain):
+        assert a["fidelity"] <= b["fidelity"] * (1 + 1e-6), (a, b)
+        if a["dc_steps"] is not None:
+            assert a["learned"] or a["rejected_candidate_fidelity"] is not None
+    _, hs, _ = LE.fista_tv_run(rc, L, g.vol_shape, 8, 0.3, 10, ph, None, cuda, learned=dca, learned_every=8,
+                               feedback=False)
+    assert len(hs) == 8 and hs[-1]["dc_steps"] is not None and all(h["dc_steps"] is None for h in hs[:-1])
