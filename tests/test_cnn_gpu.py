@@ -23,7 +23,7 @@ def _data(shape, seed):
     return x, t
 
 
-@pytest.mark.parametrize("shape", [(6, 7, 16), (5, 9, 24)])
+@pytest.mark.parametrize("shape", [(6, 7, 16), (5, 9, 24), (5, 17, 40), (9, 10, 72)])
 def test_forward_matches_oracle(cuda, shape):
     import torch
     x, _ = _data(shape, 1)
@@ -45,9 +45,9 @@ def test_forward_matches_oracle(cuda, shape):
         assert e_64 < 3e-2
 
 
-def test_gradients_match_oracle(cuda):
+@pytest.mark.parametrize("shape", [(6, 5, 16), (5, 17, 40)])
+def test_gradients_match_oracle(cuda, shape):
     import torch
-    shape = (6, 5, 16)
     x, t = _data(shape, 2)
     p = CN.init_params(4).astype(np.float64)
     ws, bs = CR.unflatten(p)
@@ -62,6 +62,33 @@ def test_gradients_match_oracle(cuda):
         ew, eb = rel_l2(wg[l], wr[l]), rel_l2(bg[l], br[l])
         print(f"layer {l + 1}: dW rel L2 {ew:.2e}, db rel L2 {eb:.2e}")
         assert ew < 2e-2 and eb < 2e-2
+
+
+def test_forward_at_evaluation_size_matches_oracle_on_crops(cuda):
+    """512x512x64 forward (the evaluation size, 65536 tiles of each layer's
+    kernel): the network output over a 22^3 crop in the interior and a 16^3
+    crop at the volume's corner, against the oracle run on the crop alone.  A
+    margin of 5 voxels (the 5 layers' receptive-field radius) on every side that
+    is not the volume's own boundary is left out of the comparison."""
+    import torch
+    shape = (512, 512, 64)
+    rng = np.random.default_rng(8)
+    x = rng.random(shape, dtype=np.float32)
+    p = CN.init_params(6).astype(np.float64)
+    p[-1] = 0.05
+    ws, bs = CR.unflatten(p)
+    net = CN.CNN(shape, training=False, params=p)
+    out = net(torch.from_numpy(x.reshape(-1)).cuda()).cpu().numpy().reshape(shape).astype(np.float64)
+    for (y0, x0, z0), c, keep in (((300, 123, 21), 22, (slice(5, 17),) * 3),
+                                  ((0, 0, 0), 16, (slice(0, 11),) * 3)):
+        crop = x[y0:y0 + c, x0:x0 + c, z0:z0 + c].astype(np.float64)
+        ref, _ = CR.forward(ws, bs, crop.reshape(-1), (c, c, c), bf16=True)
+        ref = ref.reshape(c, c, c)[keep]
+        got = out[y0:y0 + c, x0:x0 + c, z0:z0 + c][keep]
+        base = crop[keep]
+        e = rel_l2((got - base).reshape(-1), (ref - base).reshape(-1))
+        print(f"512x512x64 forward, crop at {(y0, x0, z0)}: vs bf16-emulating oracle {e:.2e}")
+        assert e < 2e-3
 
 
 def test_training_reduces_denoising_loss(cuda):
