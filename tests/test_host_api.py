@@ -157,6 +157,27 @@ def test_cxx_header_reference_names(tmp_path):
     assert threw == 1
 
 
+def test_c_call_site_compiles_and_links(tmp_path):
+    """The reference's C call site (DD3Proj_gpu with its 24 arguments) compiles
+    against include/ctlab_mi355x.h with a C99 compiler, links against the
+    library, and an unsupported prjMode is reported without touching the output."""
+    import os
+    import shutil
+    import subprocess
+    gcc = shutil.which("gcc")
+    if gcc is None:
+        pytest.skip("no gcc")
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    libdir = os.path.join(root, "ctlab_amd")
+    exe = tmp_path / "c_abi"
+    subprocess.run([gcc, "-std=c99", "-Wall", "-Wextra", "-pedantic", "-Werror", "-I", os.path.join(root, "include"),
+                    os.path.join(root, "tests", "cxx", "c_abi.c"), "-L", libdir, "-lctlab_mi355x",
+                    f"-Wl,-rpath,{libdir}", "-o", str(exe)], check=True)
+    r = subprocess.run([str(exe)], capture_output=True, text=True)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert r.stdout.split()[:2] == ["2", "1"]
+
+
 def test_learning_rate_schedules():
     """The learned prox's Adam step sizes: the half-cosine decay starts at lr,
     passes the midpoint of lr and its 0.05 floor halfway and ends on the floor,
