@@ -551,7 +551,7 @@ def main():
     ap.add_argument("--problems", type=int, default=32)
     ap.add_argument("--subsets", type=int, default=10)
     ap.add_argument("--samples", type=int, default=2000)
-    ap.add_argument("--epochs", type=int, default=24)
+    ap.add_argument("--epochs", type=int, default=40)
     ap.add_argument("--lr-schedule", default="cosine", choices=("cosine", "step"),
                     help="--learned: Adam step sizes (cosine decay to 0.05x, or the last epoch at 0.3x)")
     ap.add_argument("--iters", type=int, default=60)
