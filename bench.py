@@ -9,6 +9,9 @@ value  = 2 * XN*YN*ZN*PN / (t_proj + t_back)   voxel*views/s, both operators
 roofline (HBM model): algorithmic bytes per operator 4*PN*(XN*YN*ZN + DNU*DNV)
          divided by the operator's measured time (HIP events on the launch stream).
 cpu_baseline: the FP64 CPU restatement timed on a bounded view sample.
+The FP64 oracle (oracle/) is used only by the CPU legs, cpu_baseline and
+cfg3_cpu_baseline, as checker and timed baseline; every measured figure comes
+from the library.
 """
 import argparse
 import json
@@ -442,6 +445,22 @@ def batch_bench(args):
     print(json.dumps(res))
 
 
+def cfg3_cpu_baseline(g, b, L):
+    """The CPU leg of cfg3 (checker and timed baseline, never the measured path):
+    the FP64 oracle's OS-SART (10 subsets x 20) and FISTA-TV (100 iterations,
+    FGP 10) drivers on the same data.  Returns (OS-SART image, seconds),
+    (FISTA-TV image, objective history, seconds)."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import recon_ref as RR
+    t0 = time.perf_counter()
+    xs_ref = RR.os_sart(g, b, 10, 20)
+    t_sart = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    xf_ref, obj_ref = RR.fista_tv(g, b, 100, 1e-4, 10, L)
+    t_fista = time.perf_counter() - t0
+    return (xs_ref, t_sart), (xf_ref, obj_ref, t_fista)
+
+
 def cfg3_bench(args):
     """cfg3: OS-SART (10 subsets, 20 iterations) and FISTA-TV (100 iterations, lambda 1e-4)
     on the cfg1 fan-beam geometry (256x256, 360 views, 888-cell arc detector), data = the
@@ -449,8 +468,6 @@ def cfg3_bench(args):
     line integrals would hold mass the image cannot represent) with Poisson noise at
     I0 = 1e5 and log transform; GPU results against the FP64 oracle drivers and the phantom."""
     import torch
-    sys.path.insert(0, os.path.join(ROOT, "oracle"))
-    import recon_ref as RR
     from ctlab_amd import dd
     from ctlab_amd import geometry as G
     from ctlab_amd import phantom as PH
@@ -492,11 +509,6 @@ def cfg3_bench(args):
     torch.cuda.synchronize()
     t_sart = time.perf_counter() - t0
     xs = x.cpu().numpy()
-    t0 = time.perf_counter()
-    xs_ref = RR.os_sart(g, b, 10, 20)
-    t_sart_ref = time.perf_counter() - t0
-    out["os_sart"] = {"subsets": 10, "iterations": 20, "gpu_s": t_sart, "oracle_cpu_s": t_sart_ref,
-                      "psnr_gpu": psnr(xs), "psnr_oracle": psnr(xs_ref), "rel_l2_vs_fp64_oracle": rel(xs, xs_ref)}
     # FISTA-TV, 100 iterations, lambda 1e-4 (FGP prox, 10 inner steps)
     rf = RC.Recon(plan, 1)
     rf.set_data(bt)
@@ -508,9 +520,9 @@ def cfg3_bench(args):
     torch.cuda.synchronize()
     t_fista = time.perf_counter() - t0
     xfn = xf.cpu().numpy()
-    t0 = time.perf_counter()
-    xf_ref, obj_ref = RR.fista_tv(g, b, 100, 1e-4, 10, L)
-    t_fista_ref = time.perf_counter() - t0
+    (xs_ref, t_sart_ref), (xf_ref, obj_ref, t_fista_ref) = cfg3_cpu_baseline(g, b, L)
+    out["os_sart"] = {"subsets": 10, "iterations": 20, "gpu_s": t_sart, "oracle_cpu_s": t_sart_ref,
+                      "psnr_gpu": psnr(xs), "psnr_oracle": psnr(xs_ref), "rel_l2_vs_fp64_oracle": rel(xs, xs_ref)}
     out["fista_tv"] = {"iterations": 100, "tv_lambda": 1e-4, "L": L, "gpu_s": t_fista, "oracle_cpu_s": t_fista_ref,
                        "psnr_gpu": psnr(xfn), "psnr_oracle": psnr(xf_ref), "rel_l2_vs_fp64_oracle": rel(xfn, xf_ref),
                        "objective_rel_diff_final": abs(obj[-1] - obj_ref[-1]) / obj_ref[-1],
