@@ -44,7 +44,9 @@ zprefix_kernel(const float* __restrict__ vol, T* __restrict__ cz,
 
 // Projector walk.  Block = 4 wavefronts = 4 consecutive views of one tile of
 // NU detector columns (neighbouring views read nearly the same slab rows);
-// each wavefront loops over all slabs of its view.
+// each wavefront loops over all slabs of its view.  Blocks are dispatched view
+// group fastest: the groups of one tile, which read the same slab rows, run
+// together (3 % faster than tile fastest at 120 and at 984 views).
 template <typename T, int NU>
 __global__ void __launch_bounds__(256)
 __attribute__((amdgpu_waves_per_eu(8))) dd_proj_walk(DevGeom g, const T* __restrict__ cz, float* __restrict__ prj)
@@ -52,9 +54,12 @@ __attribute__((amdgpu_waves_per_eu(8))) dd_proj_walk(DevGeom g, const T* __restr
     __shared__ T sG[4][NU * kGStride];
     __shared__ __align__(16) double sFT[4][2 * (NU + 1)];
     const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    const int p = blockIdx.y * 4 + wv;
+    // grid (tiles, view groups); linear dispatch index read with the view group fastest
+    const unsigned id = blockIdx.y * gridDim.x + blockIdx.x;
+    const int tile = (int)(id / gridDim.y), grp = (int)(id % gridDim.y);
+    const int p = grp * 4 + wv;
     if (p >= g.PN) return;
-    proj_walk_wave<T, NU>(g, gmem(cz), gmem(prj), p, blockIdx.x, threadIdx.x & 63, sG[wv], sFT[wv]);
+    proj_walk_wave<T, NU>(g, gmem(cz), gmem(prj), p, tile, threadIdx.x & 63, sG[wv], sFT[wv]);
 }
 
 // One wavefront per (view, column): v-prefix of y*L for the backprojector.
