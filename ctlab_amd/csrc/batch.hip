@@ -160,7 +160,8 @@ __global__ void __launch_bounds__(256) bk_prefix(const BatchProb* __restrict__ p
 //   x += colSum > 1e-7 ? lambda * vol / colSum : 0, then clamp at 0 if nonneg
 // (src/SARTWeighting.h:56-91, in the order of the unfused update kernel).
 template <typename T, int NM, int YSLAB>
-__global__ void __launch_bounds__(256) bk_back(const BatchProb* __restrict__ pb, Queue q, float lambda, int nonneg)
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(7)))
+bk_back(const BatchProb* __restrict__ pb, Queue q, float lambda, int nonneg)
 {
     constexpr int yslab = YSLAB;
     __shared__ T sG[4][72];

@@ -389,15 +389,39 @@ __device__ __forceinline__ void back_walk_accum(const DevGeom& g, const CTLAB_GM
     const T sgn = g.dzh < 0.0 ? T(-1) : T(1);
     const double zoff = g.zhat0 / g.dzh;
 
-    int pn = nviews > 0 ? views[0] : 0;
-    int stn = nviews > 0 ? start[(long)pn * SJ * SR + srow] : 0xFFFF;
+    // View pipeline: the view list is read three views ahead, the start
+    // columns two ahead and the v-prefix entries of a view's first two
+    // columns one ahead, so that a view's first column does not wait on
+    // memory.  They are read through the view's buffer resource, so a view
+    // whose start is 0xFFFF (skipped) reads zeros instead of out of bounds.
+    auto first_two = [&](int pq, int sq, T& a, T& b) {
+        const int asc = g.vp[pq].asc;
+        const __amdgpu_buffer_rsrc_t rsq = __builtin_amdgcn_make_buffer_rsrc(
+            (void*)(P + (long)pq * DNU * DNV), (short)0, DNU * DNV * (int)sizeof(T), kBufWord3);
+        const unsigned off = (unsigned)((asc ? sq : DNU - 1 - sq) * DNV + vl) * (unsigned)sizeof(T);
+        a = buf_load<T>(rsq, off);
+        b = buf_load<T>(rsq, off + (unsigned)((asc ? DNV : -DNV) * (int)sizeof(T)));
+    };
+    const long sstride = (long)SJ * SR;
+    int p1 = nviews > 0 ? views[0] : 0;   // views iv, iv + 1, iv + 2 at the top of iteration iv
+    int p2 = nviews > 1 ? views[1] : 0;
+    int p3 = nviews > 2 ? views[2] : 0;
+    int st1 = nviews > 0 ? start[p1 * sstride + srow] : 0xFFFF;
+    int st2 = nviews > 1 ? start[p2 * sstride + srow] : 0xFFFF;
+    T Q0 = T(0), Q1 = T(0);
+    if (nviews > 0) first_two(__builtin_amdgcn_readfirstlane(p1), __builtin_amdgcn_readfirstlane(st1), Q0, Q1);
     for (int iv = 0; iv < nviews; ++iv) {
-        const int p = __builtin_amdgcn_readfirstlane(pn);
-        const int st = __builtin_amdgcn_readfirstlane(stn);
-        if (iv + 1 < nviews) {
-            pn = views[iv + 1];
-            stn = start[(long)pn * SJ * SR + srow];
-        }
+        const int p = __builtin_amdgcn_readfirstlane(p1);
+        const int st = __builtin_amdgcn_readfirstlane(st1);
+        const T Pr0 = Q0;
+        T Pr1 = Q1;
+        p1 = p2;
+        st1 = st2;
+        p2 = p3;
+        if (iv + 1 < nviews)
+            first_two(__builtin_amdgcn_readfirstlane(p1), __builtin_amdgcn_readfirstlane(st1), Q0, Q1);
+        if (iv + 2 < nviews) st2 = start[p2 * sstride + srow];
+        if (iv + 3 < nviews) p3 = views[iv + 3];
         if (st == 0xFFFF) continue;
         const ViewParams vp = g.vp[p];
         const double sj = vp.s0 + j * g.dx;
@@ -405,10 +429,11 @@ __device__ __forceinline__ void back_walk_accum(const DevGeom& g, const CTLAB_GM
         const double inv_sj = g.invs[(long)p * g.sjs + j];
         const double inv_rs = fabs(g.dx * inv_sj);
         const double lzb = ((double)lane - vp.beta0) * inv_sj;
-        // walk-order column c = st: its record and v-prefix row; the next
-        // column's are loaded one column ahead.  The view's v-prefix rows are
-        // read through a buffer resource, so the step past either end of the
-        // view (the sentinel column) reads 0 instead of needing a clamp.
+        // walk-order column c = st: its record (its v-prefix entries and the
+        // next column's came through the pipeline); each further column's are
+        // loaded one column ahead.  The view's v-prefix rows are read through
+        // a buffer resource, so the step past either end of the view (the
+        // sentinel column) reads 0 instead of needing a clamp.
         const CTLAB_GMEM ColRec* __restrict__ rq = rec + (long)p * (DNU + 1) + st;
         const __amdgpu_buffer_rsrc_t prs = __builtin_amdgcn_make_buffer_rsrc(
             (void*)(P + (long)p * DNU * DNV), (short)0, DNU * DNV * (int)sizeof(T), kBufWord3);
@@ -419,11 +444,9 @@ __device__ __forceinline__ void back_walk_accum(const DevGeom& g, const CTLAB_GM
         if (alo >= (double)(r0 + nm)) continue;
         double ahi = vp.rbase + rs * r.khi;
         double iw = inv_rs * r.invdk;
-        const T Pr0 = buf_load<T>(prs, voff);
         rq += 1;
         voff += vstep;
         ColRec rn = rq[0];
-        T Pr1 = buf_load<T>(prs, voff);
         T D = column_delta<ZM>(g, G, Pr0, r.esdz, lzb, zoff, inv_sj, vp.beta0, vp.top_total, lane);
         T Tsum = T(0), Wprev = T(0);
         // not unrolled (each copy would carry its own column walk)

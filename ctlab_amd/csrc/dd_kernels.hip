@@ -110,7 +110,7 @@ back_start_kernel(DevGeom g, const ColRec* __restrict__ rec, unsigned short* __r
 // (a scratch volume); back_add then sums them.  One grid instead of two
 // dependent launches leaves one tail of partly filled CUs instead of two.
 template <typename T, int NM>
-__global__ void __launch_bounds__(256)
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8)))
 dd_back_walk(DevGeom g, const T* __restrict__ P, float* __restrict__ vol_y, float* __restrict__ vol_x,
              const int* __restrict__ views_y, int ny, const int* __restrict__ views_x, int nx, int nby,
              int ntr_y, int ntr_x, const ColRec* __restrict__ rec, const unsigned short* __restrict__ start,
