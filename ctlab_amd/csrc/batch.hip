@@ -430,6 +430,13 @@ ctlab_batch* ctlab_batch_create(ctlab_recon* const* recons, int n)
             set_error(CTLAB_ERR_ARG, "all reconstructions must share one device and one SAT precision");
             return nullptr;
         }
+        if (is_blocked(r->full->in)) {
+            set_error(CTLAB_ERR_GEOMETRY,
+                      "reconstruction %d: the batch takes volumes of at most %d slices and detectors of at most "
+                      "%d rows (ZN %d, DNV %d); run it with ctlab_os_sart",
+                      i, kBlock, kBlock, r->full->in.ZN, r->full->in.DNV);
+            return nullptr;
+        }
         for (const ctlab_plan* S : r->sub)
             if (S->sat_bits != r->full->sat_bits) {
                 set_error(CTLAB_ERR_ARG, "reconstruction %d: subset plans differ in SAT precision", i);

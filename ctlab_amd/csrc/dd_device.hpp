@@ -226,8 +226,9 @@ __device__ __forceinline__ void proj_walk_wave(const DevGeom& g, const CTLAB_GME
             }
         }
         if (defer_top) {
-            // lane s < nu: top boundary of column step s (its table is region s)
-            if ((actk >> lane) & 1u) {
+            // lane s < nu: top boundary of column step s (its table is region s);
+            // the lane test keeps the shift below 32 bits
+            if (lane < nu && ((actk >> lane) & 1u)) {
                 const double t = g.cone ? sj * tau_w : 1.0;
                 acc_top += lds_interp(G + lane * kGStride, vp.beta0 + t * ztop, ZN) * qj;
             }

@@ -26,6 +26,7 @@
 // slab orientation, so no atomics are needed.
 #include "dd_device.hpp"
 
+#include <algorithm>
 #include <type_traits>
 
 namespace ctlab {
@@ -144,9 +145,50 @@ __global__ void __launch_bounds__(256) back_add(float* __restrict__ vol, const f
         vol[i] += add[i];
 }
 
+// Row-block staging for volumes deeper than 64 slices and detectors with more
+// than 64 rows: copy a block of nd consecutive elements out of every row of
+// length ns (voxel columns, detector columns), and write one back.
+__global__ void __launch_bounds__(256) rows_gather(const float* __restrict__ src, float* __restrict__ dst, long nrows,
+                                                   int ns, int off, int nd)
+{
+    const long n = nrows * nd;
+    for (long i = blockIdx.x * 256L + threadIdx.x; i < n; i += (long)gridDim.x * 256) {
+        const long r = i / nd;
+        dst[i] = src[r * ns + off + (i - r * nd)];
+    }
+}
+
+__global__ void __launch_bounds__(256) rows_scatter(const float* __restrict__ src, float* __restrict__ dst, long nrows,
+                                                    int ns, int nd, int off, int add)
+{
+    const long n = nrows * ns;
+    for (long i = blockIdx.x * 256L + threadIdx.x; i < n; i += (long)gridDim.x * 256) {
+        const long r = i / ns;
+        const long o = r * nd + off + (i - r * ns);
+        const float v = src ? src[i] : 0.0f;
+        dst[o] = add ? dst[o] + v : v;
+    }
+}
+
 // ---------------------------------------------------------------------------
 // launchers
 // ---------------------------------------------------------------------------
+static unsigned stream_grid(long n) { return (unsigned)std::max<long>(1, std::min<long>((n + 255) / 256, 8192)); }
+
+hipError_t launch_rows_gather(const float* src, float* dst, long nrows, int ns, int off, int nd, hipStream_t s)
+{
+    hipLaunchKernelGGL(rows_gather, dim3(stream_grid(nrows * nd)), dim3(256), 0, s, src, dst, nrows, ns, off, nd);
+    return hipGetLastError();
+}
+
+hipError_t launch_rows_scatter(const float* src, float* dst, long nrows, int ns, int nd, int off, int add,
+                               hipStream_t s)
+{
+    hipLaunchKernelGGL(rows_scatter, dim3(stream_grid(nrows * ns)), dim3(256), 0, s, src, dst, nrows, ns, nd, off,
+                       add);
+    return hipGetLastError();
+}
+
 template <typename T, int NU>
 static void launch_proj_walk(const DevGeom& g, const T* cz, float* d_prj, hipStream_t s)
 {

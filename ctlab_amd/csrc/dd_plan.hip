@@ -101,10 +101,6 @@ static int build_tables(ctlab_plan* P)
         set_error(CTLAB_ERR_ARG, "non-finite geometry");
         return -1;
     }
-    if (g.ZN > 64 || g.DNV > 64) {
-        set_error(CTLAB_ERR_GEOMETRY, "ZN (%d) and DNV (%d) must be <= 64", g.ZN, g.DNV);
-        return -1;
-    }
     if (!g.cone && g.ZN != g.DNV) {
         set_error(CTLAB_ERR_ARG, "multi-slice mode needs ZN == DNV");
         return -1;
@@ -113,7 +109,10 @@ static int build_tables(ctlab_plan* P)
     std::vector<double> xb(DNU + 1), yb(DNU + 1), zb(DNV + 1);
     boundaries(DNU + 1, g.xds.data(), xb.data());
     boundaries(DNU + 1, g.yds.data(), yb.data());
-    if (g.cone) boundaries(DNV + 1, g.zds.data(), zb.data());
+    if (g.cone) {
+        if (g.zbnd.size() == (size_t)DNV + 1) zb = g.zbnd;
+        else boundaries(DNV + 1, g.zds.data(), zb.data());
+    }
 
     P->zhat.resize(DNV + 1);
     P->ez.resize(DNV);
@@ -269,6 +268,11 @@ void free_plan(ctlab_plan* P)
     int cur = 0;
     hipGetDevice(&cur);
     hipSetDevice(P->device);
+    for (auto& pt : P->parts) free_plan(pt.plan);
+    P->parts.clear();
+    if (P->sub_shared) P->d_sub_vol = P->d_sub_prj = nullptr;
+    if (P->d_sub_vol) hipFree(P->d_sub_vol);
+    if (P->d_sub_prj) hipFree(P->d_sub_prj);
     if (P->ws_shared) P->d_cz = P->d_P = nullptr;
     void* ptrs[] = {P->d_vp, P->d_kap, P->d_tau, P->d_colfac, P->d_exy2, P->d_invdk, P->d_esdz, P->d_invs, P->d_zhat, P->d_ez, P->d_rec, P->d_start,
                     P->d_rowfac, P->d_mask, P->d_views, P->d_cz, P->d_P};
@@ -278,31 +282,70 @@ void free_plan(ctlab_plan* P)
     delete P;
 }
 
-static int alloc_workspace(ctlab_plan* P)
+// Workspace elements a plan needs: its integral image and projection v-prefix,
+// or, for a blocked plan, those of its largest part (the parts run one after
+// another on one workspace).
+static void ws_need(const GeomIn& in, size_t& ncz, size_t& nP)
 {
-    const size_t es = P->sat_bits == 32 ? 4 : 8;
-    const size_t nprj = (size_t)P->in.PN * P->in.DNU * P->in.DNV;
+    GeomIn g;
+    g.XN = in.XN; g.YN = in.YN;
+    g.ZN = std::min(in.ZN, kBlock);
+    ncz = cz_elems(g);
+    nP = (size_t)in.PN * in.DNU * std::min(in.DNV, kBlock);
+}
+
+// (Re)allocates the integral-image and v-prefix workspaces for `bits`.  Both
+// new buffers are allocated before the old ones are released, so on failure
+// the plan keeps its previous workspace and precision unchanged.
+static int alloc_workspace(ctlab_plan* P, int bits)
+{
+    const size_t es = bits == 32 ? 4 : 8;
+    size_t ncz = 0, nP = 0;
+    ws_need(P->in, ncz, nP);
+    void *cz = nullptr, *pp = nullptr;
+    if (hip_check(hipMalloc(&cz, ncz * es), "hipMalloc(integral image)")) return -1;
+    if (hip_check(hipMalloc(&pp, nP * es), "hipMalloc(projection prefix)")) {
+        hipFree(cz);
+        return -1;
+    }
     if (!P->ws_shared) {
         if (P->d_cz) hipFree(P->d_cz);
         if (P->d_P) hipFree(P->d_P);
     }
-    P->d_cz = P->d_P = nullptr;
+    P->d_cz = cz;
+    P->d_P = pp;
     P->ws_shared = false;
-    if (hip_check(hipMalloc(&P->d_cz, cz_elems(P->in) * es), "hipMalloc(integral image)")) return -1;
-    if (hip_check(hipMalloc(&P->d_P, nprj * es), "hipMalloc(projection prefix)")) return -1;
+    P->sat_bits = bits;
     P->ws_elem = es;
+    P->ws_cz_cap = ncz;
+    P->ws_P_cap = nP;
     return 0;
 }
 
+// W's workspace can serve P: same device and precision, large enough
 static bool can_share(const ctlab_plan* P, const ctlab_plan* W)
 {
-    return W && W->device == P->device && W->sat_bits == P->sat_bits && W->in.XN == P->in.XN &&
-           W->in.YN == P->in.YN && W->in.ZN == P->in.ZN && W->in.DNU == P->in.DNU && W->in.DNV == P->in.DNV &&
-           W->in.PN >= P->in.PN && W->d_cz && W->d_P;
+    if (!W || !W->d_cz || !W->d_P || W->device != P->device || W->sat_bits != P->sat_bits) return false;
+    size_t ncz = 0, nP = 0;
+    ws_need(P->in, ncz, nP);
+    return W->ws_cz_cap >= ncz && W->ws_P_cap >= nP;
 }
+
+static void borrow_workspace(ctlab_plan* P, const ctlab_plan* W)
+{
+    P->d_cz = W->d_cz;
+    P->d_P = W->d_P;
+    P->ws_elem = W->ws_elem;
+    P->ws_cz_cap = W->ws_cz_cap;
+    P->ws_P_cap = W->ws_P_cap;
+    P->ws_shared = true;
+}
+
+static ctlab_plan* finish_blocked(ctlab_plan* P, int gpunum, const ctlab_plan* share_ws);
 
 ctlab_plan* finish_plan(ctlab_plan* P, int gpunum, const ctlab_plan* share_ws)
 {
+    if (is_blocked(P->in)) return finish_blocked(P, gpunum, share_ws);
     if (build_tables(P)) { delete P; return nullptr; }
     int ndev = 0;
     if (hip_check(hipGetDeviceCount(&ndev), "hipGetDeviceCount")) { delete P; return nullptr; }
@@ -320,9 +363,7 @@ ctlab_plan* finish_plan(ctlab_plan* P, int gpunum, const ctlab_plan* share_ws)
         upload(&P->d_esdz, P->esdz) || upload(&P->d_zhat, P->zhat) || upload(&P->d_invs, P->invs) ||
         upload(&P->d_ez, P->ez) || upload(&P->d_rowfac, P->rowfac) || upload(&P->d_mask, P->in.mask) ||
         upload(&P->d_views, views) || upload(&P->d_rec, P->rec) ||
-        (can_share(P, share_ws) ? (P->d_cz = share_ws->d_cz, P->d_P = share_ws->d_P, P->ws_elem = share_ws->ws_elem,
-                                   P->ws_shared = true, 0)
-                                : alloc_workspace(P))) {
+        (can_share(P, share_ws) ? (borrow_workspace(P, share_ws), 0) : alloc_workspace(P, P->sat_bits))) {
         free_plan(P);
         return nullptr;
     }
@@ -339,6 +380,223 @@ ctlab_plan* finish_plan(ctlab_plan* P, int gpunum, const ctlab_plan* share_ws)
     P->rec.clear();
     P->rec.shrink_to_fit();
     return P;
+}
+
+// ---------------------------------------------------------------------------
+// Blocked plans: ZN > 64 or DNV > 64.  The projection of slice block c onto
+// row block b uses only those slices and rows, so A = sum over (c, b) of the
+// block operators A_cb, each a plan of at most 64 slices and 64 rows: its
+// volume is the slice block in place (image centre moved to the block's
+// centre), its detector the row block with the row boundaries of the whole
+// detector.  Pairs whose rays cannot meet are left out (their A_cb is 0: every
+// row boundary maps below the slice block or above it, where the row
+// differences cancel).  Multi-slice fan beam: slice block c meets row block c.
+// ---------------------------------------------------------------------------
+
+// true if, in some view, a ray through row block [v0, v0+nv] can cross the
+// slice block [z0, z0+nz] (lattice units; conservative by one slice)
+static bool pair_meets(const GeomIn& g, const std::vector<double>& zhat, int z0, int nz, int v0, int nv)
+{
+    const double Xmin = g.cx - g.XN * g.dx * 0.5, Ymin = g.cy - g.YN * g.dx * 0.5;
+    const double Zmin = g.cz - g.ZN * g.dz * 0.5;
+    const double za = zhat[v0], zb = zhat[v0 + nv];
+    for (int p = 0; p < g.PN; ++p) {
+        const double th = regularize_angle(g.angs[p]);
+        const double c = std::cos(th), s = std::sin(th);
+        const double Sx = g.x0 * c - g.y0 * s, Sy = g.x0 * s + g.y0 * c;
+        const int yslab = std::fabs(g.cy - Sy) >= std::fabs(g.cx - Sx);
+        const int NS = yslab ? g.YN : g.XN;
+        const double s0 = (yslab ? Ymin - Sy : Xmin - Sx) + 0.5 * g.dx, s1 = s0 + (NS - 1) * g.dx;
+        double emin = HUGE_VAL, emax = -HUGE_VAL;
+        for (int u = 0; u < g.DNU; ++u) {
+            const double X = g.xds[u] * c - g.yds[u] * s, Y = g.xds[u] * s + g.yds[u] * c;
+            const double es = yslab ? Y - Sy : X - Sx;
+            emin = std::min(emin, es);
+            emax = std::max(emax, es);
+        }
+        if (!(emin * emax > 0) || !(s0 * s1 > 0)) return true;   // no sign-definite bound: keep
+        // z magnification t = s / e over slabs and columns, then z = beta0 + t * zhat
+        double tlo = HUGE_VAL, thi = -HUGE_VAL;
+        for (double sv : {s0, s1})
+            for (double ev : {emin, emax}) {
+                tlo = std::min(tlo, sv / ev);
+                thi = std::max(thi, sv / ev);
+            }
+        const double beta0 = (g.z0 + g.zpos[p] - Zmin) / g.dz;
+        const double zlo = beta0 + std::min(std::min(tlo * za, tlo * zb), std::min(thi * za, thi * zb));
+        const double zhi = beta0 + std::max(std::max(tlo * za, tlo * zb), std::max(thi * za, thi * zb));
+        if (zhi > z0 - 1.0 && zlo < z0 + nz + 1.0) return true;
+    }
+    return false;
+}
+
+static ctlab_plan* finish_blocked(ctlab_plan* P, int gpunum, const ctlab_plan* share_ws)
+{
+    const GeomIn& g = P->in;
+    if (g.DNU <= 0 || g.DNV <= 0 || g.PN <= 0 || g.XN <= 0 || g.YN <= 0 || g.ZN <= 0 || !(g.dx > 0) ||
+        !(g.dz > 0)) {
+        set_error(CTLAB_ERR_ARG, "sizes and pitches must be positive");
+        delete P;
+        return nullptr;
+    }
+    if (!g.cone && g.ZN != g.DNV) {
+        set_error(CTLAB_ERR_ARG, "multi-slice mode needs ZN == DNV");
+        delete P;
+        return nullptr;
+    }
+    int ndev = 0;
+    if (hip_check(hipGetDeviceCount(&ndev), "hipGetDeviceCount")) { delete P; return nullptr; }
+    if (gpunum < 0 || gpunum >= ndev) {
+        set_error(CTLAB_ERR_ARG, "gpunum %d out of range (%d devices)", gpunum, ndev);
+        delete P;
+        return nullptr;
+    }
+    P->device = gpunum;
+    if (hip_check(hipSetDevice(gpunum), "hipSetDevice")) { delete P; return nullptr; }
+    const int DNV = g.DNV;
+    std::vector<double> zb(DNV + 1), zhat(DNV + 1);
+    if (g.cone) {
+        if (g.zbnd.size() == (size_t)DNV + 1) zb = g.zbnd;
+        else boundaries(DNV + 1, g.zds.data(), zb.data());
+        for (int v = 0; v <= DNV; ++v) zhat[v] = (zb[v] - g.z0) / g.dz;
+    }
+    // one workspace for every part, and the staging buffers
+    bool ok = true;
+    if (can_share(P, share_ws)) borrow_workspace(P, share_ws);
+    else ok = !alloc_workspace(P, P->sat_bits);
+    // a slice block's backprojection, a row block's projection (both are also
+    // where parts that add into an output block put their share)
+    const size_t sub_vol = (size_t)g.XN * g.YN * std::min(g.ZN, kBlock);
+    const size_t sub_prj = (size_t)g.PN * g.DNU * std::min(DNV, kBlock);
+    if (ok && share_ws && share_ws->d_sub_vol && share_ws->d_sub_prj && share_ws->device == P->device &&
+        (size_t)share_ws->in.XN * share_ws->in.YN >= (size_t)g.XN * g.YN &&
+        (size_t)share_ws->in.PN * share_ws->in.DNU >= (size_t)g.PN * g.DNU &&
+        std::min(share_ws->in.ZN, kBlock) >= std::min(g.ZN, kBlock) &&
+        std::min(share_ws->in.DNV, kBlock) >= std::min(DNV, kBlock)) {
+        P->d_sub_vol = share_ws->d_sub_vol;
+        P->d_sub_prj = share_ws->d_sub_prj;
+        P->sub_shared = true;
+    } else if (ok) {
+        ok = !hip_check(hipMalloc((void**)&P->d_sub_vol, sub_vol * sizeof(float)), "hipMalloc(slice block)") &&
+             !hip_check(hipMalloc((void**)&P->d_sub_prj, sub_prj * sizeof(float)), "hipMalloc(row block)");
+    }
+    const double Zmin = g.cz - g.ZN * g.dz * 0.5;
+    const int nzb = (g.ZN + kBlock - 1) / kBlock, nvb = (DNV + kBlock - 1) / kBlock;
+    for (int c = 0; ok && c < nzb; ++c) {
+        const int z0 = c * kBlock, nz = std::min(kBlock, g.ZN - z0);
+        for (int b = 0; ok && b < nvb; ++b) {
+            const int v0 = b * kBlock, nv = std::min(kBlock, DNV - v0);
+            if (g.cone ? !pair_meets(g, zhat, z0, nz, v0, nv) : c != b) continue;
+            ctlab_plan* S = new ctlab_plan();
+            S->in = g;
+            S->in.ZN = nz;
+            S->in.DNV = nv;
+            S->in.zds.assign(g.zds.begin() + v0, g.zds.begin() + v0 + nv);
+            if (g.cone) {
+                S->in.cz = Zmin + (z0 + 0.5 * nz) * g.dz;
+                S->in.zbnd.assign(zb.begin() + v0, zb.begin() + v0 + nv + 1);
+            }
+            S->sat_bits = P->sat_bits;
+            S->tile_nu = P->tile_nu;
+            S->tile_nm = P->tile_nm;
+            S = finish_plan(S, gpunum, P);
+            if (!S) { ok = false; break; }
+            P->parts.push_back({S, c, b, z0, nz, v0, nv});
+        }
+    }
+    if (!ok) {
+        free_plan(P);
+        return nullptr;
+    }
+    if (!P->parts.empty()) {
+        P->views_y = P->parts[0].plan->views_y;
+        P->views_x = P->parts[0].plan->views_x;
+    }
+    return P;
+}
+
+static void sync_part(const ctlab_plan* P, ctlab_plan* S)
+{
+    S->tile_nu = P->tile_nu;
+    S->tile_nm = P->tile_nm;
+}
+
+// projection of a blocked plan: slice block by slice block (one gather each),
+// each part's rows written into its row block (the first part of a row block
+// writes, later ones add); row blocks no part reaches are zeroed
+static int blocked_proj(ctlab_plan* P, const float* d_vol, float* d_prj, hipStream_t s)
+{
+    const GeomIn& in = P->in;
+    const long ncol = (long)in.XN * in.YN, nrow = (long)in.PN * in.DNU;
+    const int nvb = (in.DNV + kBlock - 1) / kBlock;
+    std::vector<char> done(nvb, 0);
+    int cur = -1;
+    for (auto& pt : P->parts) {
+        sync_part(P, pt.plan);
+        const float* src = d_vol;
+        if (pt.nz != in.ZN) {
+            if (pt.zb != cur && hip_check(launch_rows_gather(d_vol, P->d_sub_vol, ncol, in.ZN, pt.z0, pt.nz, s),
+                                          "slice block gather"))
+                return t_err;
+            cur = pt.zb;
+            src = P->d_sub_vol;
+        }
+        const bool first = !done[pt.vb];
+        if (first && pt.nv == in.DNV) {
+            if (plan_proj(pt.plan, src, d_prj, s)) return t_err;
+        } else {
+            if (plan_proj(pt.plan, src, P->d_sub_prj, s) ||
+                hip_check(launch_rows_scatter(P->d_sub_prj, d_prj, nrow, pt.nv, in.DNV, pt.v0, !first, s),
+                          "row block scatter"))
+                return t_err;
+        }
+        done[pt.vb] = 1;
+    }
+    for (int b = 0; b < nvb; ++b)
+        if (!done[b] && hip_check(launch_rows_scatter(nullptr, d_prj, nrow, std::min(kBlock, in.DNV - b * kBlock),
+                                                      in.DNV, b * kBlock, 0, s), "row block clear"))
+            return t_err;
+    return CTLAB_OK;
+}
+
+// backprojection of a blocked plan: row block by row block (one gather each),
+// each part's slices written into its slice block (first writes, later add)
+static int blocked_back(ctlab_plan* P, const float* d_prj, float* d_vol, hipStream_t s)
+{
+    const GeomIn& in = P->in;
+    const long ncol = (long)in.XN * in.YN, nrow = (long)in.PN * in.DNU;
+    const int nzb = (in.ZN + kBlock - 1) / kBlock, nvb = (in.DNV + kBlock - 1) / kBlock;
+    std::vector<char> done(nzb, 0);
+    for (int b = 0; b < nvb; ++b) {
+        bool gathered = false;
+        for (auto& pt : P->parts) {
+            if (pt.vb != b) continue;
+            sync_part(P, pt.plan);
+            const float* src = d_prj;
+            if (pt.nv != in.DNV) {
+                if (!gathered && hip_check(launch_rows_gather(d_prj, P->d_sub_prj, nrow, in.DNV, pt.v0, pt.nv, s),
+                                           "row block gather"))
+                    return t_err;
+                gathered = true;
+                src = P->d_sub_prj;
+            }
+            const bool first = !done[pt.zb];
+            if (first && pt.nz == in.ZN) {
+                if (plan_back(pt.plan, src, d_vol, s)) return t_err;
+            } else {
+                if (plan_back(pt.plan, src, P->d_sub_vol, s) ||
+                    hip_check(launch_rows_scatter(P->d_sub_vol, d_vol, ncol, pt.nz, in.ZN, pt.z0, !first, s),
+                              "slice block scatter"))
+                    return t_err;
+            }
+            done[pt.zb] = 1;
+        }
+    }
+    for (int c = 0; c < nzb; ++c)
+        if (!done[c] && hip_check(launch_rows_scatter(nullptr, d_vol, ncol, std::min(kBlock, in.ZN - c * kBlock), in.ZN,
+                                                      c * kBlock, 0, s), "slice block clear"))
+            return t_err;
+    return CTLAB_OK;
 }
 
 static std::vector<double> to_d(const float* p, int n)
@@ -401,7 +659,12 @@ int plan_proj(ctlab_plan* P, const float* d_vol, float* d_prj, hipStream_t s)
         set_error(CTLAB_ERR_ARG, "NULL plan or buffer");
         return CTLAB_ERR_ARG;
     }
+    if (!P->d_cz || !P->d_P) {
+        set_error(CTLAB_ERR_ARG, "plan has no workspace (an earlier allocation failed)");
+        return CTLAB_ERR_ARG;
+    }
     DeviceGuard dg(P->device);
+    if (is_blocked(P->in)) return blocked_proj(P, d_vol, d_prj, s);
     const DevGeom g = P->dev();
     hipError_t e = P->sat_bits == 32 ? launch_proj<float>(g, d_vol, P->d_cz, d_prj, s)
                                      : launch_proj<double>(g, d_vol, P->d_cz, d_prj, s);
@@ -415,7 +678,12 @@ int plan_back(ctlab_plan* P, const float* d_prj, float* d_vol, hipStream_t s)
         set_error(CTLAB_ERR_ARG, "NULL plan or buffer");
         return CTLAB_ERR_ARG;
     }
+    if (!P->d_cz || !P->d_P) {
+        set_error(CTLAB_ERR_ARG, "plan has no workspace (an earlier allocation failed)");
+        return CTLAB_ERR_ARG;
+    }
     DeviceGuard dg(P->device);
+    if (is_blocked(P->in)) return blocked_back(P, d_prj, d_vol, s);
     const DevGeom g = P->dev();
     const int ny = (int)P->views_y.size(), nx = (int)P->views_x.size();
     const int* vy = P->d_views;
@@ -515,8 +783,11 @@ int ctlab_plan_set_sat_bits(ctlab_plan* plan, int bits)
     }
     if (plan->sat_bits == bits) return CTLAB_OK;
     DeviceGuard dg(plan->device);
-    plan->sat_bits = bits;
-    if (alloc_workspace(plan)) return t_err;
+    if (alloc_workspace(plan, bits)) return t_err;
+    for (auto& pt : plan->parts) {   // the parts run on the blocked plan's workspace
+        pt.plan->sat_bits = bits;
+        borrow_workspace(pt.plan, plan);
+    }
     return CTLAB_OK;
 }
 
@@ -612,8 +883,7 @@ int ctlab_plan_views(const ctlab_plan* plan, int* n_yslab, int* n_xslab)
 long long ctlab_plan_workspace_bytes(const ctlab_plan* plan)
 {
     if (!plan) return -1;
-    const size_t nprj = (size_t)plan->in.PN * plan->in.DNU * plan->in.DNV;
-    return (long long)((cz_elems(plan->in) + nprj) * plan->ws_elem);
+    return (long long)((plan->ws_cz_cap + plan->ws_P_cap) * plan->ws_elem);
 }
 
 void DD3Proj_gpu(float x0, float y0, float z0, int DNU, int DNV, float* xds, float* yds, float* zds,

@@ -17,7 +17,16 @@ struct GeomIn {
     double dx, dz;
     std::vector<unsigned char> mask;  // empty = none
     int cone;
+    std::vector<double> zbnd;         // DNV+1 row boundaries when given (row blocks of a larger
+                                      // detector); empty: DD3Boundaries of zds
 };
+
+// The kernels hold one voxel column (z) and one detector column (v) in the 64
+// lanes of a wavefront.  Larger volumes and detectors are split into blocks of
+// at most kBlock slices and kBlock rows; a plan over them is the sum of one
+// sub-plan per (slice block, row block) pair that rays can join.
+constexpr int kBlock = 64;
+inline bool is_blocked(const GeomIn& in) { return in.ZN > kBlock || in.DNV > kBlock; }
 
 // integral-image elements: the volume plus two rows of padding for the
 // projector's read-ahead past a slab's last row (an x-slab row is XN*ZN
@@ -30,6 +39,11 @@ inline size_t cz_elems(const GeomIn& in)
 
 template <typename T>
 hipError_t launch_proj(const DevGeom& g, const float* d_vol, void* d_cz, float* d_prj, hipStream_t s);
+// dst[r*nd + k] = src[r*ns + off + k] for rows r < nrows, k < nd
+hipError_t launch_rows_gather(const float* src, float* dst, long nrows, int ns, int off, int nd, hipStream_t s);
+// dst[r*nd + off + k] = (add ? dst[...] : 0) + (src ? src[r*ns + k] : 0) for r < nrows, k < ns
+hipError_t launch_rows_scatter(const float* src, float* dst, long nrows, int ns, int nd, int off, int add,
+                               hipStream_t s);
 // d_scratch: XN*YN*ZN floats of work space (the x-slab part before it is added)
 template <typename T>
 hipError_t launch_back(const DevGeom& g, const float* d_prj, void* d_P, float* d_vol, float* d_scratch,
@@ -62,8 +76,22 @@ struct ctlab_plan {
     void* d_P = nullptr;    // projection v-prefix (nprj elements)
     int sat_bits = 64;
     size_t ws_elem = 0;
+    size_t ws_cz_cap = 0, ws_P_cap = 0;   // workspace capacity in elements
     bool ws_shared = false; // d_cz / d_P belong to another plan (subset plans of a reconstruction)
     int tile_nu = 8, tile_nm = 16;     // the autotuner's choice at 512x512x64, 984 views
+
+    // Blocked plans (ZN or DNV above kBlock): no tables of their own; parts[i]
+    // covers slices [z0, z0+nz) and rows [v0, v0+nv).  All parts run on one
+    // workspace (the first part's) and on the staging buffers below.
+    struct Part {
+        ctlab_plan* plan;
+        int zb, vb;          // block indices
+        int z0, nz, v0, nv;
+    };
+    std::vector<Part> parts;
+    float* d_sub_vol = nullptr;   // XN*YN*min(ZN, kBlock): one slice block
+    float* d_sub_prj = nullptr;   // PN*DNU*min(DNV, kBlock): one row block
+    bool sub_shared = false;      // staging buffers belong to another plan
 
     ctlab::DevGeom dev() const
     {

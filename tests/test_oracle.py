@@ -252,3 +252,38 @@ def test_boundaries_rule():
     # src/utilities.cuh:413-433
     assert np.allclose(O.boundaries(np.array([0.0, 1.0, 3.0])), [-0.5, 0.5, 2.0, 4.0])
     assert np.allclose(O.boundaries(np.array([2.0])), [1.5, 2.5])
+
+
+def test_block_decomposition_sums_to_full_operator():
+    """The identity blocked plans rest on: the projection of a volume of more
+    than 64 slices onto a detector of more than 64 rows is the sum, over
+    (slice block, row block) pairs, of the projection of the slice block alone
+    (a volume of its own, centred on the block) onto the row block alone.  The
+    row centres are exact binary fractions, so the blocks' boundary rule gives
+    the full detector's boundaries.  Also the backprojection, block by block."""
+    g = G.circular_cone(XN=12, YN=10, ZN=150, dx=1.0, dz=0.5, PN=10, DNU=20, DNV=140, col_size=1.5,
+                        row_size=0.5, sid=60.0, sdd=100.0, ang0=0.3)
+    assert np.all(np.diff(g.zds.astype(np.float64)) == 0.5)
+    rng = np.random.default_rng(30)
+    vol = rng.random(g.vol_shape)
+    y = rng.random(g.nprj())
+    full_p = O.proj(g, vol).reshape(g.PN, g.DNU, g.DNV)
+    full_b = O.back(g, y).reshape(g.vol_shape)
+    yb = y.reshape(g.PN, g.DNU, g.DNV)
+    sum_p = np.zeros_like(full_p)
+    sum_b = np.zeros_like(full_b)
+    zmin = g.imgZCenter - g.ZN * g.dz / 2
+    for z0 in range(0, g.ZN, 64):
+        nz = min(64, g.ZN - z0)
+        for v0 in range(0, g.DNV, 64):
+            nv = min(64, g.DNV - v0)
+            sub = G.ConeGeometry(**{k: getattr(g, k) for k in g.__dataclass_fields__})
+            sub.ZN, sub.DNV = nz, nv
+            sub.imgZCenter = zmin + (z0 + 0.5 * nz) * g.dz
+            sub.zds = g.zds[v0:v0 + nv].copy()
+            part = O.proj(sub, np.ascontiguousarray(vol[:, :, z0:z0 + nz]))
+            sum_p[:, :, v0:v0 + nv] += part.reshape(g.PN, g.DNU, nv)
+            back = O.back(sub, np.ascontiguousarray(yb[:, :, v0:v0 + nv]))
+            sum_b[:, :, z0:z0 + nz] += back.reshape(g.YN, g.XN, nz)
+    assert rel_l2(sum_p, full_p) < 1e-12
+    assert rel_l2(sum_b, full_b) < 1e-12
