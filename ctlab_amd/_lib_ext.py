@@ -29,6 +29,7 @@ def declare(L):
         "ctlab_soft_threshold": ([_fp, f, ll, _vp], i),
         "ctlab_axpby": ([f, _fp, f, _fp, ll, _vp], i),
         "ctlab_cg_beta": ([_fp, _fp, _fp, i, ll, _dp, _vp], i),
+        "ctlab_cg_beta_static": ([_fp, _fp, _fp, i, ll, _vp, _vp], i),
         "ctlab_tv_prox": ([_fp, _fp, f, i, i, i, i, _fp, _vp], i),
         "ctlab_batch_create": ([C.POINTER(_vp), i], _vp),
         "ctlab_batch_destroy": ([_vp], None),

@@ -15,6 +15,7 @@
 #include <cmath>
 #include <functional>
 #include <cstring>
+#include <mutex>
 #include <vector>
 
 #include "../../include/ctlab_mi355x.h"
@@ -509,6 +510,38 @@ int ctlab_soft_threshold(float* d_x, float g, long long n, void* stream)
     return chk(rk_soft_threshold(d_x, g, n, (hipStream_t)stream), "soft threshold") ? ctlab_last_error() : 0;
 }
 
+}  // extern "C"
+
+// Scratch of ctlab_cg_beta_static: the FP64 partial sums of the beta
+// reduction (two per block of its fixed grid).  One array per GPU, held until
+// the process ends; every call overwrites all of it before reading it, so
+// nothing needs clearing, and calls that overlap on one GPU race on it.
+static constexpr int kBetaScratchDoubles = 2048;
+static __device__ __align__(256) double scratch[kBetaScratchDoubles];
+
+namespace {
+std::mutex g_scratch_mu;
+std::vector<double*> g_scratch_ptr;   // device address of `scratch`, per device
+
+// device address of `scratch` on the current device (looked up once per device,
+// with that device current); null with the error recorded on failure
+double* scratch_on_current_device()
+{
+    int dev = 0;
+    if (chk(hipGetDevice(&dev), "hipGetDevice")) return nullptr;
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if ((int)g_scratch_ptr.size() <= dev) g_scratch_ptr.resize(dev + 1, nullptr);
+    if (!g_scratch_ptr[dev]) {
+        void* ptr = nullptr;
+        if (chk(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)), "hipGetSymbolAddress(scratch)")) return nullptr;
+        g_scratch_ptr[dev] = static_cast<double*>(ptr);
+    }
+    return g_scratch_ptr[dev];
+}
+}  // namespace
+
+extern "C" {
+
 int ctlab_cg_beta(const float* d_gk, const float* d_gk1, const float* d_d, int cases, long long n, double* h_numden,
                   void* stream)
 {
@@ -531,6 +564,31 @@ int ctlab_cg_beta(const float* d_gk, const float* d_gk1, const float* d_d, int c
     hipFree(part);
     hipFree(red);
     return CTLAB_OK;
+}
+
+// ctlab_cg_beta without any allocation: the partial sums live in the
+// file-scope `scratch` array and the two sums go to the caller's device
+// buffer d_numden, stream-ordered (no synchronisation), so the call can be
+// captured in a HIP graph.  The scratch need is fixed (two doubles per block
+// of the reduction grid), whatever n.
+int ctlab_cg_beta_static(const float* d_gk, const float* d_gk1, const float* d_d, int cases, long long n,
+                         double* d_numden, void* stream)
+{
+    clear_error();
+    if (!d_gk || !d_gk1 || !d_d || !d_numden || cases < 0 || cases > 4 || n < 0) {
+        set_error(CTLAB_ERR_ARG, "bad beta arguments");
+        return CTLAB_ERR_ARG;
+    }
+    if (red_scratch_doubles() > kBetaScratchDoubles) {
+        set_error(CTLAB_ERR_ARG, "reduction needs %d doubles of scratch, the static array holds %d",
+                  red_scratch_doubles(), kBetaScratchDoubles);
+        return CTLAB_ERR_ARG;
+    }
+    double* part = scratch_on_current_device();
+    if (!part) return ctlab_last_error();
+    return chk(rk_beta(d_gk, d_gk1, d_d, cases, 0, n, part, d_numden, (hipStream_t)stream), "beta")
+               ? ctlab_last_error()
+               : CTLAB_OK;
 }
 
 }  // extern "C"

@@ -268,6 +268,15 @@ int ctlab_soft_threshold(float* d_x, float g, long long n, void* stream);
 int ctlab_axpby(float a, const float* d_x, float b, float* d_y, long long n, void* stream);
 int ctlab_cg_beta(const float* d_gk, const float* d_gk1, const float* d_d, int cases, long long n,
                   double* h_numden, void* stream);
+/* No counterpart in the reference.  ctlab_cg_beta for callers that cannot
+ * allocate device memory (code under HIP graph capture, serving loops,
+ * harnesses that forbid hipMalloc): no allocation, no synchronisation; the two
+ * FP64 sums go to d_numden (2 doubles, device memory), stream-ordered.  Its
+ * scratch (the reduction's partial sums, 16 KiB, fixed whatever n) is one
+ * static array per GPU shared by every call on that GPU: calls that overlap on
+ * one GPU race. */
+int ctlab_cg_beta_static(const float* d_gk, const float* d_gk1, const float* d_d, int cases, long long n,
+                         double* d_numden, void* stream);
 /* isotropic TV prox (FGP, iters steps) on a (Z,X,Y) volume; d_work: 9*XN*YN*ZN floats */
 int ctlab_tv_prox(const float* d_b, float* d_x, float lam, int iters, int XN, int YN, int ZN, float* d_work,
                   void* stream);
