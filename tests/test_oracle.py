@@ -287,3 +287,22 @@ def test_block_decomposition_sums_to_full_operator():
             sum_b[:, :, z0:z0 + nz] += back.reshape(g.YN, g.XN, nz)
     assert rel_l2(sum_p, full_p) < 1e-12
     assert rel_l2(sum_b, full_b) < 1e-12
+
+
+def test_ddm_ed_proj_matches_branchless_flat():
+    """DDM_ED_proj (flat detector at y = -O2D, cell pitch detSize; src/DDM_proj.h:
+    42-55) equals the branchless oracle on the same flat-detector fan geometry.
+    The cell centres are exact binary fractions, so DD3Boundaries of the centres
+    gives the flat detector's own boundaries and every cell agrees (a detector
+    placed at +O2D, or a shifted centre index, fails)."""
+    S2O, S2D, DN, pitch = 541.0, 949.0, 600, 0.5
+    g = G.fan_multislice(XN=64, YN=64, SLN=1, dx=4.0, PN=36, DNU=DN, col_size=pitch, sid=S2O, sdd=S2D,
+                         arc=False, ang0=0.01)
+    assert np.all(np.diff(g.xds.astype(np.float64)) == pitch)
+    rng = np.random.default_rng(8)
+    img = rng.random(g.nvox())
+    for dt in (np.float64, np.float32):
+        ed = O.ddm("ed_proj", S2O, S2D - S2O, 256.0, 256.0, pitch, (DN - 1) / 2.0, 64, 64, DN,
+                   g.angs.astype(np.float64), img, dtype=dt)
+        tol = 1e-12 if dt == np.float64 else 2e-6
+        assert rel_l2(ed, O.proj(g, img)) < tol

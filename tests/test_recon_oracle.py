@@ -124,3 +124,50 @@ def test_fista_tv_and_cg_reduce_the_objective():
     assert obj[-1] < 0.05 * obj[0]
     _, res = RR.cg(g, b, 6, 0)
     assert res[-1] < res[0]
+
+
+def test_soft_threshold_is_the_prox_of_g_abs():
+    """_softThreshold_functor (src/utilities.hpp:1097-1117) is the proximal map
+    of g|x|: argmin_x 0.5 (x - v)^2 + g |x|, found here by brute force on a
+    fine grid around each v (a sign slip or a missing dead zone fails)."""
+    v = np.array([-3.1, -0.7, -0.2999, -0.3, 0.0, 0.1, 0.3, 0.31, 1.0, 4.25])
+    for g in (0.0, 0.3, 1.5):
+        got = O.soft_threshold(v, g)
+        for vi, xi in zip(v, got):
+            grid = np.linspace(vi - g - 1.0, vi + g + 1.0, 400001)
+            obj = 0.5 * (grid - vi) ** 2 + g * np.abs(grid)
+            best = grid[np.argmin(obj)]
+            assert abs(xi - best) <= 2.5 * (grid[1] - grid[0]), (g, vi, xi, best)
+    assert np.all(O.soft_threshold(np.array([-0.3, 0.3, 0.2]), 0.3) == 0.0)
+
+
+def test_ossart_update_special_cases():
+    """ossart_update_functor (src/utilities.hpp:1202-1223): (img + lambda*cor/weg)*msk,
+    0 where IS_ZERO(weg), i.e. |weg| < 1e-9 (strict, :316-320)."""
+    img = np.array([1.5, -2.0, 3.0, 0.25, 7.0, 1.0, 2.0])
+    cor = np.array([0.5, 4.0, -1.0, 8.0, 2.0, 3.0, -6.0])
+    weg = np.array([1.0, 1.0, 1.0, 5e-10, -5e-10, 1e-9, -2.0])
+    msk = np.array([1.0, 1.0, 0.0, 1.0, 1.0, 1.0, 0.5])
+    lam = 0.75
+    got = O.ossart_update(img, cor, weg, msk, lam)
+    assert got[0] == img[0] + lam * cor[0] and got[1] == img[1] + lam * cor[1]   # weg = msk = 1
+    assert got[2] == 0.0                                                        # masked out
+    assert got[3] == 0.0 and got[4] == 0.0                                      # |weg| < 1e-9
+    assert got[5] == (img[5] + lam * cor[5] / 1e-9)                             # 1e-9 itself is not zero
+    assert got[6] == (img[6] + lam * cor[6] / -2.0) * 0.5
+
+
+def test_tv_value_closed_forms():
+    """Isotropic TV with forward differences, zero at the far faces: a unit
+    step along z gives one unit jump per voxel column; a plane a*x + b*y gives
+    sqrt(a^2 + b^2) inside, |b| on the last x face, |a| on the last y face and
+    0 on the last edge (an anisotropic sum or a wrapped face fails)."""
+    XN, YN, ZN = 5, 4, 6
+    vol = np.zeros((YN, XN, ZN))
+    vol[:, :, 3:] = 1.0
+    assert abs(O.tv_value(vol.reshape(-1), XN, YN, ZN) - XN * YN) < 1e-12
+    a, b = 0.3, -0.4
+    y, x = np.meshgrid(np.arange(YN), np.arange(XN), indexing="ij")
+    plane = np.repeat((a * x + b * y)[:, :, None], ZN, axis=2)
+    exp = (np.hypot(a, b) * (XN - 1) * (YN - 1) + abs(b) * (YN - 1) + abs(a) * (XN - 1)) * ZN
+    assert abs(O.tv_value(plane.reshape(-1), XN, YN, ZN) - exp) < 1e-12
