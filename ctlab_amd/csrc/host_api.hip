@@ -184,8 +184,10 @@ int ctlab_getVolZIdxPair(const float* zPos, int PN, int prjStart, int prjEnd, fl
         set_error(CTLAB_ERR_ARG, "need zPos, 0 <= prjStart < prjEnd <= PN, DNV, ZN > 0 and dz > 0");
         return CTLAB_ERR_ARG;
     }
-    // detector extent below and above the source plane, at the detector
-    const float below = (detCntIdxV + 0.5f) * detStpZ;
+    // detector extent below and above the source plane, at the detector; the
+    // reference's (detCntIdxV + 0.5) * detStpZ is a double expression, rounded
+    // to float once when stored
+    const float below = (float)(((double)detCntIdxV + 0.5) * (double)detStpZ);
     const float above = (float)DNV * detStpZ - below;
     const float lo = zPos[prjStart] - below;
     const float hi = zPos[prjEnd - 1] + above;
@@ -210,10 +212,12 @@ int ctlab_getPrjIdxPair(const float* zPos, int PN, int objStart, int objEnd, flo
             set_error(CTLAB_ERR_ARG, "zPos must be sorted ascending (view %d)", p);
             return CTLAB_ERR_ARG;
         }
-    // z extent of the sub-volume and the source positions whose detector can reach it
-    const float lowV = ((float)objStart - objCntIdxZ - 0.5f) * dz;
+    // z extent of the sub-volume and the source positions whose detector can
+    // reach it; the terms with 0.5 are double expressions in the reference,
+    // rounded to float once when stored
+    const float lowV = (float)(((double)((float)objStart - objCntIdxZ) - 0.5) * (double)dz);
     const float highV = lowV + (float)(objEnd - objStart) * dz;
-    const float below = (detCntIdxV + 0.5f) * detStpZ;
+    const float below = (float)(((double)detCntIdxV + 0.5) * (double)detStpZ);
     const float above = (float)DNV * detStpZ - below;
     const float srcLo = lowV - above;
     const float srcHi = highV + below;

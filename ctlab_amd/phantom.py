@@ -107,6 +107,35 @@ def line_integral(ells, p0, p1):
     return total
 
 
+# CTLAB's SheppLoganPhantom (src/Ellipsoid.h:252-317): ten ellipses on the unit
+# square [-1/2, 1/2]^2 (the classic table with lengths and positions halved),
+# as (centre x, centre y, semi-axis x, semi-axis y, rotation in degrees,
+# intensity); the intensities add where ellipses overlap.
+_SHEPP_LOGAN = (
+    (0.0, 0.0, 0.345, 0.46, 0.0, 1.0),
+    (0.0, -0.0092, 0.3312, 0.437, 0.0, -0.8),
+    (0.11, 0.0, 0.055, 0.155, -18.0, -0.2),
+    (-0.11, 0.0, 0.08, 0.205, 18.0, -0.2),
+    (0.0, 0.175, 0.105, 0.125, 0.0, 0.1),
+    (0.0, 0.05, 0.023, 0.023, 0.0, 0.1),
+    (0.0, -0.05, 0.023, 0.023, 0.0, 0.1),
+    (-0.04, -0.3025, 0.023, 0.0115, 0.0, 0.1),
+    (0.0, -0.303, 0.0115, 0.0115, 0.0, 0.1),
+    (0.03, -0.3025, 0.0115, 0.023, 0.0, 0.1),
+)
+
+
+def shepp_logan(size_mm, mu=1.0, height_mm=None):
+    """CTLAB's Shepp-Logan phantom scaled to a square of side size_mm, intensities
+    times mu (e.g. 0.02 /mm for water-like soft tissue), in the ellipsoid format
+    of this module.  Without height_mm the ellipses are infinite cylinders along
+    z (a 2-D phantom on every slice, as the reference's); with it, every
+    ellipse becomes an ellipsoid of z semi-axis height_mm / 2 about z = 0."""
+    c = np.inf if height_mm is None else 0.5 * float(height_mm)
+    return np.array([(x * size_mm, y * size_mm, 0.0, a * size_mm, b * size_mm, c, np.deg2rad(r), m * mu)
+                     for (x, y, a, b, r, m) in _SHEPP_LOGAN], dtype=np.float64)
+
+
 def add_poisson_noise(prj, I0, seed):
     """counts ~ Poisson(I0 exp(-p)); returns -log(max(counts,1)/I0) as float32."""
     rng = np.random.Generator(np.random.PCG64(seed))

@@ -6,6 +6,8 @@ The range helpers are pinned by what they are for: every slice that the
 oracle's dense system matrix shows a view range touching lies inside the slice
 range returned for it, and every view touching a slice range lies inside the
 view range returned for it."""
+import math
+
 import numpy as np
 import pytest
 
@@ -192,3 +194,66 @@ def test_learning_rate_schedules():
     assert LE.learning_rate(lr, "step", 0, T, 0, 1) == lr
     with pytest.raises(ValueError):
         LE.learning_rate(lr, "linear", 0, T, 0, 1)
+
+
+def _vol_pair_ref(z_lo, z_hi, cnt, stp, DNV, ocnt, dz, ZN, promote=True):
+    """getVolZIdxPair (src/utilities.cuh:478-499) with T = float: the terms with
+    0.5 are double expressions rounded to float once when stored."""
+    f32, f64 = np.float32, np.float64
+    lower = f32((f64(cnt) + 0.5) * f64(stp)) if promote else f32((cnt + f32(0.5)) * stp)
+    upper = f32(f32(f32(DNV) * stp) - lower)
+    sp, ep = f32(z_lo - lower), f32(z_hi + upper)
+    s = math.floor(f32(f32(f32(sp / dz) + ocnt) - f32(1)))
+    e = math.ceil(f32(f32(f32(ep / dz) + ocnt) + f32(1))) + 1
+    return min(max(s, 0), ZN), min(max(e, 0), ZN)
+
+
+def _prj_pair_ref(zpos, o0, o1, ocnt, dz, cnt, stp, DNV, promote=True):
+    """getPrjIdxPair (src/utilities.cuh:505-532) with T = float."""
+    f32, f64 = np.float32, np.float64
+    d = f32(f32(o0) - ocnt)
+    lowv = f32((f64(d) - 0.5) * f64(dz)) if promote else f32((d - f32(0.5)) * dz)
+    highv = f32(lowv + f32(f32(o1 - o0) * dz))
+    lower = f32((f64(cnt) + 0.5) * f64(stp)) if promote else f32((cnt + f32(0.5)) * stp)
+    upper = f32(f32(f32(DNV) * stp) - lower)
+    lo, hi = f32(lowv - upper), f32(highv + lower)
+    s = int(np.searchsorted(zpos, lo, side="right")) - 1
+    e = int(np.searchsorted(zpos, hi, side="right")) + 2
+    PN = zpos.size
+    return min(max(s, 0), PN), min(max(e, 0), PN)
+
+
+def test_range_helpers_round_like_the_reference():
+    """The range helpers evaluate (detCntIdxV + 0.5) * detStpZ and
+    (ObjZIdx_Start - objCntIdxZ - 0.5) * dz in double and store them as float,
+    as the reference's float instantiation does; the cases are built so that the
+    all-float evaluation lands on the other side of a floor / ceil / sorted
+    position for some of them."""
+    f32 = np.float32
+    rng = np.random.default_rng(40)
+    n_vol = n_prj = differ_vol = differ_prj = 0
+    while (n_vol < 300 or differ_vol < 3) and n_vol < 20000:
+        cnt, stp = f32(rng.uniform(0, 128)), f32(rng.uniform(0.2, 2.0))
+        lp = f32((np.float64(cnt) + 0.5) * np.float64(stp))
+        dz = f32(rng.choice([0.25, 0.5, 1.0]))
+        zp = f32(lp + f32(int(rng.integers(-6, 6))) * dz)
+        ocnt = f32(rng.choice([10.0, 31.5, 63.5]))
+        zpos = np.array([zp, zp + f32(3.0)], dtype=np.float32)
+        got = dd.getVolZIdxPair(zpos, 0, 2, cnt, stp, 64, ocnt, dz, 4096)
+        assert got == _vol_pair_ref(zpos[0], zpos[1], cnt, stp, 64, ocnt, dz, 4096)
+        differ_vol += got != _vol_pair_ref(zpos[0], zpos[1], cnt, stp, 64, ocnt, dz, 4096, promote=False)
+        n_vol += 1
+    while (n_prj < 300 or differ_prj < 3) and n_prj < 20000:
+        ocnt, dz = f32(rng.uniform(0, 100)), f32(rng.uniform(0.3, 2.0))
+        o0 = int(rng.integers(0, 100))
+        cnt, stp = f32(rng.uniform(0, 64)), f32(rng.uniform(0.2, 2.0))
+        lowv = f32((np.float64(f32(f32(o0) - ocnt)) - 0.5) * np.float64(dz))
+        upper = f32(f32(f32(64) * stp) - f32((np.float64(cnt) + 0.5) * np.float64(stp)))
+        base = f32(lowv - upper)
+        zpos = np.sort(np.concatenate([base + f32(0.5) * np.arange(-8, 8, dtype=np.float32),
+                                       np.array([base], dtype=np.float32)])).astype(np.float32)
+        got = dd.getPrjIdxPair(zpos, o0, o0 + 5, ocnt, dz, 200, cnt, stp, 64)
+        assert got == _prj_pair_ref(zpos, o0, o0 + 5, ocnt, dz, cnt, stp, 64)
+        differ_prj += got != _prj_pair_ref(zpos, o0, o0 + 5, ocnt, dz, cnt, stp, 64, promote=False)
+        n_prj += 1
+    assert differ_vol >= 3 and differ_prj >= 3, (differ_vol, differ_prj)
