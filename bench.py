@@ -82,6 +82,149 @@ def cpu_baseline(g, vol, prj_gpu, back_fn, nviews):
     }, dict(par, views_checked=int(views.size)), (sub, p_ref, b_ref, views)
 
 
+def cpu_model():
+    """The host CPU's model name (lscpu, else /proc/cpuinfo)."""
+    import subprocess
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=10).stdout
+        for line in out.splitlines():
+            if line.startswith("Model name:"):
+                return line.split(":", 1)[1].strip()
+    except (OSError, subprocess.SubprocessError):
+        pass
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
+
+
+def cfg1_cpu_baseline(reps=5):
+    """CTLAB's CPU distance-driven path at cfg1 full size: DDM_EA_proj / DDM_EA_bproj
+    (equiangular arc, src/DDM_proj.h:208-221, src/DDM_back.h:189-209) and DDM_ED_proj /
+    DDM_ED_bproj (flat detector, src/DDM_proj.h:42-55, src/DDM_back.h:153-173), as
+    restated in oracle/ddm_core.inc (the reference's bodies are not in its tree), in
+    double and float, on all the threads OpenMP is given and on one; each the median
+    of `reps` runs after one warm-up.  256x256 image of 10 seeded random ellipses (seed
+    1), 360 views over 360 degrees, 888 cells of 1.0239 mm, SID 541, SDD 949."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+    from ctlab_amd import phantom as PH
+    S2O, S2D, DN, PN = 541.0, 949.0, 888, 360
+    img = PH.voxelize(PH.random_ellipses(1, 10, 256, 256, 1.0), 256, 256, 1, 1.0, 1.0).reshape(-1)
+    angs = np.arange(PN, dtype=np.float64) * (2.0 * np.pi / PN)
+    all_threads = O.max_threads()
+    res = {}
+    for threads in (all_threads, 1):
+        O.set_threads(threads)
+        per = {}
+        for kind, pitch in (("EA", 1.0239 / S2D), ("ED", 1.0239)):
+            for dt, tag in ((np.float64, "f64"), (np.float32, "f32")):
+                ts = {}
+                for op in ("proj", "bproj"):
+                    data = img if op == "proj" else np.ones(PN * DN)
+                    run = lambda: O.ddm(f"{kind.lower()}_{op}", S2O, S2D - S2O, 256.0, 256.0, pitch, (DN - 1) / 2.0,
+                                        256, 256, DN, angs, data, dtype=dt)
+                    run()
+                    tt = []
+                    for _ in range(reps):
+                        t0 = time.perf_counter()
+                        run()
+                        tt.append(time.perf_counter() - t0)
+                    ts[op + "_s"] = float(np.median(tt))
+                per[f"{kind}_{tag}"] = ts
+        res[str(threads)] = per
+    O.set_threads(all_threads)
+    return {"threads_all": all_threads, "by_threads": res, "reps": reps, "statistic": "median after 1 warm-up",
+            "voxel_views": 256 * 256 * PN,
+            "workload": "cfg1 full size: 256x256 image, 360 views, 888 arc (EA) / flat (ED) cells, SID 541, SDD 949"}
+
+
+def cfg1_gpu_times(reps=20):
+    """The same cfg1 operators on the GPU (DD2_multiGPU's plan: branchless fan beam),
+    device-resident, HIP events on the launch stream; milliseconds per call."""
+    import torch
+    from ctlab_amd import dd
+    from ctlab_amd import geometry as G
+    from ctlab_amd import phantom as PH
+    g = G.fan_multislice(XN=256, YN=256, SLN=1, dx=1.0, PN=360, DNU=888, col_size=1.0239, sid=541.0, sdd=949.0)
+    img = PH.voxelize(PH.random_ellipses(1, 10, 256, 256, 1.0), 256, 256, 1, 1.0, 1.0)
+    pl = dd.Plan(g)
+    x = torch.from_numpy(img).cuda()
+    p = pl.proj(x)
+    b = pl.back(p)
+    s = torch.cuda.current_stream()
+    e = [torch.cuda.Event(enable_timing=True) for _ in range(3)]
+    torch.cuda.synchronize()
+    e[0].record(s)
+    for _ in range(reps):
+        pl.proj(x, out=p)
+    e[1].record(s)
+    for _ in range(reps):
+        pl.back(p, out=b)
+    e[2].record(s)
+    torch.cuda.synchronize()
+    return {"proj_ms": e[0].elapsed_time(e[1]) / reps, "back_ms": e[1].elapsed_time(e[2]) / reps}
+
+
+def host_abi_times(g, vol_np, reps=3):
+    """Wall time of the drop-in DD3Proj_gpu / DD3Back_gpu calls (host pointers, the
+    caller's preallocated arrays) at this geometry: the first call after the plan
+    cache is cleared (builds the plan) and the median of `reps` repeated calls;
+    beside them the pageable host<->device copies of the same buffers alone."""
+    import ctypes as C
+    import torch
+    from ctlab_amd import _lib
+    L = _lib.lib()
+    f32 = lambda a: np.ascontiguousarray(a, dtype=np.float32)
+    fp = lambda a: a.ctypes.data_as(C.POINTER(C.c_float))
+    xds, yds, zds, angs, zpos = f32(g.xds), f32(g.yds), f32(g.zds), f32(g.angs), f32(g.zpos)
+    hvol = f32(vol_np).reshape(-1)
+    hprj = np.zeros(g.nprj(), dtype=np.float32)
+    hback = np.zeros(g.nvox(), dtype=np.float32)
+    geo = (g.x0, g.y0, g.z0, g.DNU, g.DNV, fp(xds), fp(yds), fp(zds), g.imgXCenter, g.imgYCenter, g.imgZCenter,
+           fp(angs), fp(zpos), g.PN, g.XN, g.YN, g.ZN)
+
+    def proj():
+        L.DD3Proj_gpu(*geo, fp(hvol), fp(hprj), g.dx, g.dz, None, 0, 0)
+
+    def back():
+        L.DD3Back_gpu(*geo, fp(hback), fp(hprj), g.dx, g.dz, None, 0, 0, 0)
+
+    def timed(fn):
+        t0 = time.perf_counter()
+        fn()
+        _lib.check()
+        return (time.perf_counter() - t0) * 1e3
+    res = {}
+    for name, fn in (("DD3Proj_gpu", proj), ("DD3Back_gpu", back)):
+        L.ctlab_plan_cache_clear()
+        cold = timed(fn)
+        warm = float(np.median([timed(fn) for _ in range(reps)]))
+        res[name] = {"cold_ms": cold, "warm_ms": warm}
+    dv = torch.empty(g.nvox(), device="cuda")
+    dp = torch.empty(g.nprj(), device="cuda")
+    tv, tp = torch.from_numpy(hvol), torch.from_numpy(hprj)
+
+    def copy_ms(dst, src):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t0) * 1e3
+    res["pageable_copies_ms"] = {
+        "h2d_volume": float(np.median([copy_ms(dv, tv) for _ in range(reps)])),
+        "d2h_projections": float(np.median([copy_ms(tp, dp) for _ in range(reps)])),
+        "h2d_projections": float(np.median([copy_ms(dp, tp) for _ in range(reps)])),
+        "d2h_volume": float(np.median([copy_ms(tv, dv) for _ in range(reps)]))}
+    res["cache_entries"] = int(L.ctlab_plan_cache_size())
+    L.ctlab_plan_cache_clear()
+    return res
+
+
 def errors(p_gpu, p_ref, b_gpu, b_ref):
     """relative L2 and max-abs error of a GPU projection / backprojection against FP64 references"""
     rel = lambda a, r: float(np.linalg.norm(a - r) / np.linalg.norm(r))
@@ -344,6 +487,104 @@ def learned_bench(args):
     print(json.dumps(res))
 
 
+WEIGHTS = os.path.join(ROOT, "ctlab_amd", "weights", "cnn_prox_cfg5")
+
+
+def learned_eval(seeds=(7, 8, 9), iters=150, every=5, dc_max=20, target=40.0):
+    """cfg5 with the committed trained network (ctlab_amd/weights/cnn_prox_cfg5.npy,
+    trained by `bench.py --learned --save-params`; its TV weight and training record
+    in the .json beside it): on each 512x512x64 evaluation phantom (120 views, I0 1e4,
+    seeds 7-9), plain FISTA-TV against the learned path, each at a data residual no
+    higher than plain's.
+
+    plain:   FISTA-TV (the TV weight chosen on a held-out phantom), time of the first
+             iterate at >= 40 dB PSNR (over the slices every view covers) and of its
+             best iterate; f_ref = ||Ax-b||^2 of the plain iterate at 40 dB (at its best
+             PSNR when it never reaches 40 dB).
+    learned: N plain iterations, then one learned step: x + net(x), CG on ||Ax-b||^2
+             until the residual is at or below min(f_ref, plain iterate's), at most
+             dc_max steps, kept only if that holds (the safeguard, evaluated inside the
+             timed step); time = plain's time at N + the step's.  The earliest N
+             (every 5th) whose output reaches 40 dB (and, separately, plain's best PSNR)
+             gives the learned time.
+    Iteration times are GPU work, synchronised; PSNR and the monitoring of
+    residuals the methods do not use are outside the timed regions."""
+    import torch
+    from ctlab_amd import dd
+    from ctlab_amd import learned as LE
+    from ctlab_amd import recon as RC
+    meta_path, params_path = WEIGHTS + ".json", WEIGHTS + ".npy"
+    params = np.load(params_path, allow_pickle=False).astype(np.float32)
+    with open(meta_path) as f:
+        meta = json.load(f)
+    lam = float(meta["tv_lambda"])
+    dev = torch.device("cuda:0")
+    ge = LE.cone_geometry(512, 512, 64, 120)
+    pe = dd.Plan(ge)
+    re_ = RC.Recon(pe, 1)
+    ze = LE.covered_slices(ge)
+    net = LE.ScaledCNN(ge.vol_shape, params)
+    dca = LE.DataConsistent(net, re_, to_fidelity=True, m_max=dc_max)
+    out = {"weights": os.path.relpath(params_path, ROOT), "tv_lambda": lam, "iters": iters, "learned_every_n": every,
+           "dc_max": dc_max, "target_db": target, "psnr_slices": [ze.start, ze.stop], "seeds": {}}
+    xc = torch.empty(ge.vol_shape, device=dev)
+    for i, sd in enumerate(seeds):
+        ph = LE.make_problem(ge, pe, re_, sd, 1e4, dev)
+        if i == 0:
+            L = re_.lipschitz(30) * 1.01
+            # untimed warm-up at this size (first launches, allocator, clocks)
+            LE.fista_tv_run(re_, L, ge.vol_shape, 20, lam, 10, None, None, dev, learned=dca, learned_every=10,
+                            feedback=False)
+        _, hp, snaps = LE.fista_tv_run(re_, L, ge.vol_shape, iters, lam, 10, ph, ze, dev,
+                                       snaps=set(range(every, iters + 1, every)))
+        t40 = LE.time_to_psnr(hp, target)
+        best = max(hp, key=lambda h: h["psnr"])
+        f_ref = t40[2] if t40[0] is not None else best["fidelity"]
+        rows = []
+        for N in sorted(snaps):
+            f_plain = hp[N - 1]["fidelity"]
+            goal = min(f_ref, f_plain)
+
+            def step():
+                net(snaps[N], xc)
+                steps, _ = re_.cg_to_fidelity(xc, goal * (1.0 - 1e-5), dc_max)
+                return steps, re_.data_fidelity(xc)
+            (cg_steps, f_c), dt = LE._timed(step)
+            rows.append({"N": N, "t_s": hp[N - 1]["t"] + dt, "step_s": dt, "cg_steps": cg_steps,
+                         "fidelity": f_c, "kept": f_c <= goal, "psnr": LE.psnr(xc, ph, ze),
+                         "plain_psnr": hp[N - 1]["psnr"], "plain_fidelity": f_plain})
+        del snaps
+
+        def first(level):
+            ok = [r for r in rows if r["kept"] and r["psnr"] >= level]
+            return min(ok, key=lambda r: r["t_s"]) if ok else None
+        l40, lbest = first(target), first(best["psnr"])
+        rec = {"plain_t_to_40dB_s": t40[0], "plain_iter_to_40dB": t40[1], "plain_best_psnr": best["psnr"],
+               "plain_best_iter": best["iter"], "plain_t_best_s": best["t"], "f_ref": f_ref,
+               "learned_t_to_40dB_s": l40["t_s"] if l40 else None, "learned_N_to_40dB": l40["N"] if l40 else None,
+               "learned_t_to_plain_best_s": lbest["t_s"] if lbest else None,
+               "learned_N_to_plain_best": lbest["N"] if lbest else None,
+               "speedup_to_40dB": (t40[0] / l40["t_s"]) if (t40[0] and l40) else None,
+               "speedup_to_plain_best": (best["t"] / lbest["t_s"]) if lbest else None,
+               "learned_best_psnr": max((r["psnr"] for r in rows if r["kept"]), default=None),
+               "rows": rows}
+        out["seeds"][str(sd)] = rec
+        print(f"cfg5 seed {sd}: plain 40 dB at {t40[0]} s (best {best['psnr']:.2f} dB at {best['t']:.3f} s); "
+              f"learned 40 dB at {rec['learned_t_to_40dB_s']} s, plain's best at {rec['learned_t_to_plain_best_s']} s",
+              file=sys.stderr, flush=True)
+        del ph
+    reach = [r for r in out["seeds"].values() if r["plain_t_to_40dB_s"] is not None]
+    out["plain_reaches_40dB_on"] = [k for k, r in out["seeds"].items() if r["plain_t_to_40dB_s"] is not None]
+    out["min_speedup_to_40dB_where_plain_reaches_it"] = min((r["speedup_to_40dB"] or 0.0) for r in reach) if reach else None
+    out["min_speedup_to_plain_best"] = min((r["speedup_to_plain_best"] or 0.0) for r in out["seeds"].values())
+    out["target_3x_met"] = bool(reach) and all((r["speedup_to_40dB"] or 0.0) >= 3.0 for r in reach)
+    rate, t_fwd = LE.cnn_forward_rate(ge.vol_shape, params, dev)
+    out["cnn_forward"] = {"shape": [512, 512, 64], "s": t_fwd, "tflops": rate / 1e12,
+                          "mfma_frac_of_bf16_dense_peak": rate / 2.5e15}
+    out["training"] = {k: meta[k] for k in meta if k != "tv_lambda"}
+    return out
+
+
 def cfg4_specs(n=32, seed=4):
     """n uneven problems: volume 256..512 (step 32) square x 16..64 slices, 360..984
     views, detector 512..888 x 16..64, I0 log-uniform in [1e4, 1e6]; the first two
@@ -555,6 +796,7 @@ def main():
                     help="views (spread over the rotation) of the CPU baseline and parity sample")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-tune", action="store_true")
+    ap.add_argument("--no-learned", action="store_true", help="skip the cfg5 learned-vs-plain evaluation")
     ap.add_argument("--tiles", default="", help="fixed projector,backprojector tiles (e.g. 8,8) instead of tuning")
     ap.add_argument("--learned", action="store_true", help="cfg5: learned FISTA vs plain FISTA-TV")
     ap.add_argument("--batch", action="store_true", help="cfg4: batched OS-SART over uneven problems")
@@ -707,6 +949,12 @@ def main():
         t32b.append(e32[1].elapsed_time(e32[2]))
     out["fp32_sat"] = {"ms_proj": float(np.mean(t32p)), "ms_back": float(np.mean(t32b)),
                        "voxel_views_per_s": 2.0 * vv / (1e-3 * (np.mean(t32p) + np.mean(t32b)))}
+    # the drop-in host-pointer entry points at this size (plan cache cold / warm)
+    ha = host_abi_times(g, vol_np)
+    cp = ha["pageable_copies_ms"]
+    ha["device_plus_copies_ms"] = {"DD3Proj_gpu": out["ms_proj"] + cp["h2d_volume"] + cp["d2h_projections"],
+                                   "DD3Back_gpu": out["ms_back"] + cp["h2d_projections"] + cp["d2h_volume"]}
+    out["host_abi"] = ha
     if not args.no_cpu:
         def back_fn(sub, p_np, bits=args.sat_bits):
             pl = dd.Plan(sub, sat_bits=bits)
@@ -714,11 +962,20 @@ def main():
             torch.cuda.synchronize()
             return r.cpu().numpy()
         cb, par, (sub, p_ref, b_ref, views) = cpu_baseline(g, vol_np, prj.cpu().numpy(), back_fn, args.cpu_views)
+        cb["nproc"] = os.cpu_count()
+        cb["omp_num_threads"] = os.environ.get("OMP_NUM_THREADS")
+        cb["cpu_model"] = cpu_model()
+        cb["cfg1_reference_cpu_path"] = cfg1_cpu_baseline()
+        cb["cfg1_gpu_ms"] = cfg1_gpu_times()
         out["cpu_baseline"] = cb
         out["parity"] = par
         out["fp32_sat"]["parity"] = errors(prj32.cpu().numpy()[views].reshape(-1).astype(np.float64), p_ref,
                                            back_fn(sub, prj.cpu().numpy()[views], 32).reshape(-1).astype(np.float64),
                                            b_ref)
+    if not args.no_learned and os.path.exists(WEIGHTS + ".npy"):
+        del vol, prj, bak, prj32, bak32, plan, p32
+        torch.cuda.empty_cache()
+        out["learned"] = learned_eval()
     print(json.dumps(out))
 
 

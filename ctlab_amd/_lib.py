@@ -77,6 +77,12 @@ def _declare(L):
     L.ctlab_plan_views.restype = i
     L.ctlab_plan_workspace_bytes.argtypes = [_vp]
     L.ctlab_plan_workspace_bytes.restype = C.c_longlong
+    L.ctlab_plan_cache_set_limit.argtypes = [i]
+    L.ctlab_plan_cache_set_limit.restype = i
+    L.ctlab_plan_cache_clear.argtypes = []
+    L.ctlab_plan_cache_clear.restype = None
+    L.ctlab_plan_cache_size.argtypes = []
+    L.ctlab_plan_cache_size.restype = i
     _declare_optional(L)
 
 

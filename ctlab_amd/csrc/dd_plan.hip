@@ -6,8 +6,12 @@
 #include <algorithm>
 #include <cmath>
 #include <limits>
+#include <list>
+#include <memory>
+#include <mutex>
 #include <cstdarg>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -48,6 +52,23 @@ int hip_check(hipError_t e, const char* what)
               hipGetErrorString(e));
     return -1;
 }
+
+// CTLAB's entry points return void, and its callers never read a status: a
+// failed call of one of them also prints its message to stderr (once, from the
+// outermost entry point of a nested call; CTLAB_QUIET=1 in the environment
+// turns it off).
+static thread_local int t_void_depth = 0;
+struct VoidReport {
+    const char* fn;
+    explicit VoidReport(const char* f) : fn(f) { ++t_void_depth; }
+    ~VoidReport()
+    {
+        if (--t_void_depth == 0 && t_err != CTLAB_OK) {
+            const char* q = getenv("CTLAB_QUIET");
+            if (!(q && q[0] == '1')) fprintf(stderr, "%s: error %d: %s\n", fn, t_err, t_msg);
+        }
+    }
+};
 
 // src/utilities.cuh:413-433 (DD3Boundaries)
 static void boundaries(int nb, const double* c, double* b)
@@ -699,35 +720,167 @@ int plan_back(ctlab_plan* P, const float* d_prj, float* d_vol, hipStream_t s)
     return CTLAB_OK;
 }
 
-// host-pointer round trip used by the CTLAB entry points
-static void host_call(ctlab_plan* P, float* hvol, float* hprj, bool forward)
+// ---------------------------------------------------------------------------
+// Plan cache of the host-pointer entry points.  The reference's callers call
+// DD3Proj_gpu / DD3Back_gpu once per subset and iteration with the same
+// geometry (src/DD3_GPU_recon.h:60-65); building the per-view tables, the
+// integral-image workspace and the start table each time would cost more than
+// the projection.  Each entry keeps a plan, device buffers for one volume and
+// one projection set, and a stream, keyed by every geometry argument (compared
+// byte for byte) and the device; least recently used entries are dropped
+// beyond the limit.  An entry is used by one call at a time (its own lock);
+// entries are never freed at process exit (the HIP runtime may be gone by
+// then), only by ctlab_plan_cache_clear / a lower limit.
+// ---------------------------------------------------------------------------
+struct CacheEntry {
+    std::vector<unsigned char> key;
+    int device = 0;
+    ctlab_plan* plan = nullptr;
+    float* dv = nullptr;
+    float* dp = nullptr;
+    hipStream_t s = nullptr;
+    std::mutex busy;
+    ~CacheEntry()
+    {
+        if (!plan) return;
+        DeviceGuard dg(device);
+        if (s) hipStreamSynchronize(s);
+        if (dv) hipFree(dv);
+        if (dp) hipFree(dp);
+        if (s) hipStreamDestroy(s);
+        free_plan(plan);
+    }
+};
+
+struct PlanCache {
+    std::mutex mu;
+    std::list<std::shared_ptr<CacheEntry>> lru;   // front: most recent
+    int limit = 16;   // an OS-SART caller of the host entry points cycles through its subsets
+};
+
+static PlanCache& plan_cache()
 {
-    if (!P) return;
+    static PlanCache* c = new PlanCache();   // deliberately never destroyed
+    return *c;
+}
+
+struct KeyWriter {
+    std::vector<unsigned char> b;
+    template <typename T> void put(const T& v)
+    {
+        const unsigned char* q = reinterpret_cast<const unsigned char*>(&v);
+        b.insert(b.end(), q, q + sizeof(T));
+    }
+    template <typename T> void arr(const T* a, size_t n)
+    {
+        put(n);
+        if (a && n) {
+            const unsigned char* q = reinterpret_cast<const unsigned char*>(a);
+            b.insert(b.end(), q, q + n * sizeof(T));
+        } else {
+            put((int)(a != nullptr));
+        }
+    }
+};
+
+// A plan and buffers for the key, from the cache or built by make(); null on
+// error.  The entry comes back locked by `lock`.
+template <class Make>
+static std::shared_ptr<CacheEntry> cache_get(std::vector<unsigned char>&& key, int device, Make make,
+                                             std::unique_lock<std::mutex>& lock)
+{
+    PlanCache& C = plan_cache();
+    std::shared_ptr<CacheEntry> e;
+    {
+        std::lock_guard<std::mutex> g(C.mu);
+        for (auto it = C.lru.begin(); it != C.lru.end(); ++it)
+            if ((*it)->device == device && (*it)->key == key) {
+                e = *it;
+                C.lru.splice(C.lru.begin(), C.lru, it);
+                break;
+            }
+    }
+    if (e) {
+        lock = std::unique_lock<std::mutex>(e->busy);
+        return e;
+    }
+    e = std::make_shared<CacheEntry>();
+    e->key = std::move(key);
+    e->device = device;
+    e->plan = make();
+    if (!e->plan) return nullptr;
+    DeviceGuard dg(device);
+    const size_t nvox = (size_t)e->plan->in.XN * e->plan->in.YN * e->plan->in.ZN;
+    const size_t nprj = (size_t)e->plan->in.PN * e->plan->in.DNU * e->plan->in.DNV;
+    if (hip_check(hipStreamCreateWithFlags(&e->s, hipStreamNonBlocking), "hipStreamCreate") ||
+        hip_check(hipMalloc(&e->dv, nvox * sizeof(float)), "hipMalloc(volume)") ||
+        hip_check(hipMalloc(&e->dp, nprj * sizeof(float)), "hipMalloc(projection)"))
+        return nullptr;
+    lock = std::unique_lock<std::mutex>(e->busy);
+    std::vector<std::shared_ptr<CacheEntry>> dropped;   // freed after the cache lock is released
+    {
+        std::lock_guard<std::mutex> g(C.mu);
+        C.lru.push_front(e);
+        while ((int)C.lru.size() > std::max(C.limit, 0)) {
+            dropped.push_back(C.lru.back());
+            C.lru.pop_back();
+        }
+    }
+    return e;
+}
+
+// host-pointer round trip used by the CTLAB entry points, on a cached entry:
+// H2D of the input, the operator, D2H of the output, all on the entry's stream
+static void host_call(CacheEntry& E, float* hvol, float* hprj, bool forward)
+{
+    ctlab_plan* P = E.plan;
     DeviceGuard dg(P->device);
     const size_t nvox = (size_t)P->in.XN * P->in.YN * P->in.ZN;
     const size_t nprj = (size_t)P->in.PN * P->in.DNU * P->in.DNV;
-    float *dv = nullptr, *dp = nullptr;
-    hipStream_t s = nullptr;
-    bool ok = !hip_check(hipStreamCreate(&s), "hipStreamCreate") &&
-              !hip_check(hipMalloc(&dv, nvox * sizeof(float)), "hipMalloc(volume)") &&
-              !hip_check(hipMalloc(&dp, nprj * sizeof(float)), "hipMalloc(projection)");
-    if (ok) {
-        if (forward) {
-            ok = !hip_check(hipMemcpyAsync(dv, hvol, nvox * sizeof(float), hipMemcpyHostToDevice, s), "H2D") &&
-                 plan_proj(P, dv, dp, s) == CTLAB_OK;
-        } else {
-            ok = !hip_check(hipMemcpyAsync(dp, hprj, nprj * sizeof(float), hipMemcpyHostToDevice, s), "H2D") &&
-                 plan_back(P, dp, dv, s) == CTLAB_OK;
-        }
-        if (ok) ok = !hip_check(hipStreamSynchronize(s), "kernel execution");
-        if (ok) {
-            ok = forward ? !hip_check(hipMemcpy(hprj, dp, nprj * sizeof(float), hipMemcpyDeviceToHost), "D2H")
-                         : !hip_check(hipMemcpy(hvol, dv, nvox * sizeof(float), hipMemcpyDeviceToHost), "D2H");
-        }
+    bool ok;
+    if (forward) {
+        ok = !hip_check(hipMemcpyAsync(E.dv, hvol, nvox * sizeof(float), hipMemcpyHostToDevice, E.s), "H2D") &&
+             plan_proj(P, E.dv, E.dp, E.s) == CTLAB_OK &&
+             !hip_check(hipMemcpyAsync(hprj, E.dp, nprj * sizeof(float), hipMemcpyDeviceToHost, E.s), "D2H");
+    } else {
+        ok = !hip_check(hipMemcpyAsync(E.dp, hprj, nprj * sizeof(float), hipMemcpyHostToDevice, E.s), "H2D") &&
+             plan_back(P, E.dp, E.dv, E.s) == CTLAB_OK &&
+             !hip_check(hipMemcpyAsync(hvol, E.dv, nvox * sizeof(float), hipMemcpyDeviceToHost, E.s), "D2H");
     }
-    if (dv) hipFree(dv);
-    if (dp) hipFree(dp);
-    if (s) hipStreamDestroy(s);
+    if (ok) hip_check(hipStreamSynchronize(E.s), "kernel execution");
+    else hipStreamSynchronize(E.s);
+}
+
+static std::vector<unsigned char> cone_key(float x0, float y0, float z0, int DNU, int DNV, const float* xds,
+                                           const float* yds, const float* zds, float cx, float cy, float cz,
+                                           const float* hangs, const float* hzPos, int PN, int XN, int YN, int ZN,
+                                           float dx, float dz, const byte* mask)
+{
+    KeyWriter k;
+    k.put(1);
+    k.put(x0); k.put(y0); k.put(z0); k.put(DNU); k.put(DNV); k.put(cx); k.put(cy); k.put(cz);
+    k.put(PN); k.put(XN); k.put(YN); k.put(ZN); k.put(dx); k.put(dz);
+    const bool sizes_ok = DNU > 0 && DNV > 0 && PN > 0 && XN > 0 && YN > 0;
+    k.arr(xds, sizes_ok ? (size_t)DNU : 0); k.arr(yds, sizes_ok ? (size_t)DNU : 0);
+    k.arr(zds, sizes_ok ? (size_t)DNV : 0); k.arr(hangs, sizes_ok ? (size_t)PN : 0);
+    k.arr(hzPos, sizes_ok ? (size_t)PN : 0); k.arr(mask, sizes_ok ? (size_t)XN * YN : 0);
+    return std::move(k.b);
+}
+
+static void cone_host_call(float x0, float y0, float z0, int DNU, int DNV, float* xds, float* yds, float* zds,
+                           float cx, float cy, float cz, float* hangs, float* hzPos, int PN, int XN, int YN, int ZN,
+                           float* hvol, float* hprj, float dx, float dz, byte* mask, int gpunum, bool forward)
+{
+    std::unique_lock<std::mutex> lock;
+    auto e = cache_get(cone_key(x0, y0, z0, DNU, DNV, xds, yds, zds, cx, cy, cz, hangs, hzPos, PN, XN, YN, ZN, dx,
+                                dz, mask),
+                       gpunum,
+                       [&] {
+                           return plan_create_cone(x0, y0, z0, DNU, DNV, xds, yds, zds, cx, cy, cz, hangs, hzPos,
+                                                   PN, XN, YN, ZN, dx, dz, mask, gpunum);
+                       },
+                       lock);
+    if (e) host_call(*e, hvol, hprj, forward);
 }
 
 }  // namespace ctlab
@@ -880,6 +1033,41 @@ int ctlab_plan_views(const ctlab_plan* plan, int* n_yslab, int* n_xslab)
     return CTLAB_OK;
 }
 
+int ctlab_plan_cache_set_limit(int entries)
+{
+    clear_error();
+    if (entries < 0) {
+        set_error(CTLAB_ERR_ARG, "cache limit must be >= 0");
+        return -1;
+    }
+    PlanCache& C = plan_cache();
+    std::vector<std::shared_ptr<CacheEntry>> dropped;   // freed after the cache lock is released
+    int prev;
+    {
+        std::lock_guard<std::mutex> g(C.mu);
+        prev = C.limit;
+        C.limit = entries;
+        while ((int)C.lru.size() > entries) {
+            dropped.push_back(C.lru.back());
+            C.lru.pop_back();
+        }
+    }
+    return prev;
+}
+
+void ctlab_plan_cache_clear(void)
+{
+    const int prev = ctlab_plan_cache_set_limit(0);
+    ctlab_plan_cache_set_limit(prev);
+}
+
+int ctlab_plan_cache_size(void)
+{
+    PlanCache& C = plan_cache();
+    std::lock_guard<std::mutex> g(C.mu);
+    return (int)C.lru.size();
+}
+
 long long ctlab_plan_workspace_bytes(const ctlab_plan* plan)
 {
     if (!plan) return -1;
@@ -892,6 +1080,7 @@ void DD3Proj_gpu(float x0, float y0, float z0, int DNU, int DNV, float* xds, flo
                  int gpunum, int prjMode)
 {
     clear_error();
+    VoidReport report("DD3Proj_gpu");
     if (prjMode != 0) {
         set_error(CTLAB_ERR_MODE, "prjMode %d not supported (branchless DD is mode 0)", prjMode);
         return;
@@ -900,10 +1089,8 @@ void DD3Proj_gpu(float x0, float y0, float z0, int DNU, int DNV, float* xds, flo
         set_error(CTLAB_ERR_ARG, "NULL volume or projection pointer");
         return;
     }
-    ctlab_plan* P = plan_create_cone(x0, y0, z0, DNU, DNV, xds, yds, zds, imgXCenter, imgYCenter,
-                                     imgZCenter, hangs, hzPos, PN, XN, YN, ZN, dx, dz, mask, gpunum);
-    host_call(P, hvol, hprj, true);
-    free_plan(P);
+    cone_host_call(x0, y0, z0, DNU, DNV, xds, yds, zds, imgXCenter, imgYCenter, imgZCenter, hangs, hzPos, PN, XN,
+                   YN, ZN, hvol, hprj, dx, dz, mask, gpunum, true);
 }
 
 void DD3Back_gpu(float x0, float y0, float z0, int DNU, int DNV, float* xds, float* yds, float* zds,
@@ -913,6 +1100,7 @@ void DD3Back_gpu(float x0, float y0, float z0, int DNU, int DNV, float* xds, flo
 {
     (void)squared;
     clear_error();
+    VoidReport report("DD3Back_gpu");
     if (prjMode != 0) {
         set_error(CTLAB_ERR_MODE, "prjMode %d not supported (branchless DD is mode 0)", prjMode);
         return;
@@ -921,10 +1109,8 @@ void DD3Back_gpu(float x0, float y0, float z0, int DNU, int DNV, float* xds, flo
         set_error(CTLAB_ERR_ARG, "NULL volume or projection pointer");
         return;
     }
-    ctlab_plan* P = plan_create_cone(x0, y0, z0, DNU, DNV, xds, yds, zds, imgXCenter, imgYCenter,
-                                     imgZCenter, hangs, hzPos, PN, XN, YN, ZN, dx, dz, mask, gpunum);
-    host_call(P, hvol, hprj, false);
-    free_plan(P);
+    cone_host_call(x0, y0, z0, DNU, DNV, xds, yds, zds, imgXCenter, imgYCenter, imgZCenter, hangs, hzPos, PN, XN,
+                   YN, ZN, hvol, hprj, dx, dz, mask, gpunum, false);
 }
 
 void DD2_multiGPU(float* hvol, float* hprj, const int method, const float x0, const float y0, float* xds,
@@ -933,6 +1119,7 @@ void DD2_multiGPU(float* hvol, float* hprj, const int method, const float x0, co
                   int* startIdx, const int gpuNum)
 {
     clear_error();
+    VoidReport report("DD2_multiGPU");
     if (method != 0 && method != 2) {
         set_error(CTLAB_ERR_MODE, "method %d not supported (0: branchless proj, 2: branchless back)", method);
         return;
@@ -958,10 +1145,21 @@ void DD2_multiGPU(float* hvol, float* hprj, const int method, const float x0, co
     }
     int dev = 0;
     hipGetDevice(&dev);
-    ctlab_plan* P = plan_create_fan(x0, y0, xds, yds, DNU, SLN, imgXCenter, imgYCenter, XN, YN, dx, hangs,
-                                    PN, mask, dev);
-    host_call(P, hvol, hprj, method == 0);
-    free_plan(P);
+    KeyWriter k;
+    k.put(0);
+    k.put(x0); k.put(y0); k.put(DNU); k.put(SLN); k.put(imgXCenter); k.put(imgYCenter); k.put(XN); k.put(YN);
+    k.put(dx); k.put(PN);
+    const bool sizes_ok = DNU > 0 && SLN > 0 && PN > 0 && XN > 0 && YN > 0;
+    k.arr(xds, sizes_ok ? (size_t)DNU : 0); k.arr(yds, sizes_ok ? (size_t)DNU : 0);
+    k.arr(hangs, sizes_ok ? (size_t)PN : 0); k.arr(mask, sizes_ok ? (size_t)XN * YN : 0);
+    std::unique_lock<std::mutex> lock;
+    auto e = cache_get(std::move(k.b), dev,
+                       [&] {
+                           return plan_create_fan(x0, y0, xds, yds, DNU, SLN, imgXCenter, imgYCenter, XN, YN, dx,
+                                                  hangs, PN, mask, dev);
+                       },
+                       lock);
+    if (e) host_call(*e, hvol, hprj, method == 0);
 }
 
 // ---- multi-GPU split entry points ----------------------------------------
@@ -1002,6 +1200,7 @@ void DD3Proj_multiGPU(float x0, float y0, float z0, int DNU, int DNV, float* xds
                       int prjMode, int* startPN, int gpuNum)
 {
     clear_error();
+    VoidReport report("DD3Proj_multiGPU");
     if (prjMode != 0) {
         set_error(CTLAB_ERR_MODE, "prjMode %d not supported (branchless DD is mode 0)", prjMode);
         return;
@@ -1017,6 +1216,7 @@ void DD3Back_multiGPU(float x0, float y0, float z0, int DNU, int DNV, float* xds
                       int methodId, int* startVOL, int gpuNum)
 {
     clear_error();
+    VoidReport report("DD3Back_multiGPU");
     if (methodId != 0) {
         set_error(CTLAB_ERR_MODE, "methodId %d not supported (branchless DD is mode 0)", methodId);
         return;

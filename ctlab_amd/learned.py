@@ -104,17 +104,19 @@ def covered_slices(g):
 
 
 class ScaledCNN:
-    """x -> x + net(s x)/s, the network seeing attenuation scaled by INPUT_SCALE."""
+    """x -> x + net(s x)/s, the network seeing attenuation scaled by INPUT_SCALE.
+    The network's correction is taken back out of its output exactly
+    ((s x + net) - s x), so a network that outputs 0 returns x bit for bit."""
 
     def __init__(self, shape, params, training=False):
         self.net = CN.CNN(shape, training=training, params=params)
         self.s = INPUT_SCALE
 
     def __call__(self, x, out, f_ref=None):
-        import torch
         xs = x * self.s
-        self.net(xs, out=out)
-        RC.axpby(0.0, out, 1.0 / self.s, out)
+        self.net(xs, out=out)                     # out = s x + net(s x)
+        RC.axpby(-1.0, xs, 1.0, out)              # net(s x)
+        RC.axpby(1.0, x, 1.0 / self.s, out)       # x + net(s x) / s
         return out
 
 

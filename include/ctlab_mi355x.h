@@ -34,6 +34,12 @@ void DD3Proj_gpu(float x0, float y0, float z0, int DNU, int DNV, float* xds, flo
                  int PN, int XN, int YN, int ZN, float* hvol, float* hprj, float dx, float dz,
                  byte* mask, int gpunum, int prjMode);
 
+/* DD3Proj_gpu / DD3Back_gpu / DD2_multiGPU keep, per geometry and device, the
+ * plan (per-view tables, integral-image workspace) and device buffers of their
+ * last calls: a repeated call with the same geometry runs only H2D, the
+ * operator and D2H.  ctlab_plan_cache_* (this library's own) bound or free
+ * that memory.  A failed call also prints its message to stderr, since these
+ * entry points return void (CTLAB_QUIET=1 silences it). */
 /* src/DD3_GPU_Back.h:40-44.  Exact adjoint of DD3Proj_gpu, prjMode 0;
  * `squared` is accepted and ignored (the reference calls it a fake parameter,
  * src/DD3_GPU_Back.h:38).  hvol is overwritten. */
@@ -132,6 +138,11 @@ int ctlab_plan_autotune(ctlab_plan* plan, float* d_vol, float* d_prj, float* ms_
 /* geometry queries */
 int ctlab_plan_views(const ctlab_plan* plan, int* n_yslab, int* n_xslab);
 long long ctlab_plan_workspace_bytes(const ctlab_plan* plan);
+/* plan cache of the host-pointer entry points: at most `entries` geometries
+ * (default 16; 0 frees and disables it); returns the previous limit */
+int ctlab_plan_cache_set_limit(int entries);
+void ctlab_plan_cache_clear(void);
+int ctlab_plan_cache_size(void);
 
 /* ---- reconstruction on the device (this library's own) ------------------ */
 typedef struct ctlab_recon ctlab_recon;

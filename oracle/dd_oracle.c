@@ -468,3 +468,7 @@ void ddo_orientation(const ddo_geom *g, int *out)
 #include "ddm_core.inc"
 #undef DDM_T
 #undef DDM_SUF
+
+/* OpenMP threads of the oracle's parallel loops (the CPU baseline times 1 and all) */
+void ddo_set_threads(int n) { omp_set_num_threads(n > 0 ? n : 1); }
+int ddo_max_threads(void) { return omp_get_max_threads(); }

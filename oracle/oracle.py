@@ -59,6 +59,9 @@ def _declare(L):
     L.ddo_dense.argtypes = [gp, _dp]
     L.ddo_orientation.argtypes = [gp, C.POINTER(C.c_int)]
     L.ddo_boundaries.argtypes = [C.c_int, _dp, _dp]
+    L.ddo_set_threads.argtypes = [C.c_int]
+    L.ddo_set_threads.restype = None
+    L.ddo_max_threads.restype = C.c_int
     for t, P in (("d", _dp), ("f", _fp)):
         T = C.c_double if t == "d" else C.c_float
         for nm in ("ddm_proj_pts_", "ddm_back_pts_"):
@@ -165,6 +168,15 @@ def orientation(g):
     out = np.zeros(og.s.PN, dtype=np.int32)
     lib().ddo_orientation(C.byref(og.s), out.ctypes.data_as(C.POINTER(C.c_int)))
     return out
+
+
+def set_threads(n):
+    """OpenMP threads of the oracle's parallel loops."""
+    lib().ddo_set_threads(int(n))
+
+
+def max_threads():
+    return int(lib().ddo_max_threads())
 
 
 def boundaries(c):

@@ -230,3 +230,28 @@ def test_multi_gpu_split_entry_points_equal_single_call(cuda):
     assert np.array_equal(dd.DD3Back_multiGPU(g, y, [0, 6], 2), b1)
     assert np.array_equal(dd.DD3Back_multiGPU(g, y, [0, 4, 9], 3, helical=True), b1)
     assert np.array_equal(dd.DD3Back_multiGPU(g, y, [0, 4, 8, 12], 4, helical=True), b1)
+
+
+def test_host_abi_plan_cache(cuda):
+    """The host entry points keep plans per geometry: repeated calls, a changed
+    geometry (angles, mask) and a return to the first one each give the plan
+    path's result; the limit and clear calls bound and free the cache."""
+    from ctlab_amd import _lib
+    L = _lib.lib()
+    L.ctlab_plan_cache_clear()
+    g1 = _cone(PN=20)
+    g2 = _cone(PN=20, ang0=0.4)
+    g3 = _cone(PN=20, mask=(np.arange(64 * 56) % 7 != 0).astype(np.uint8))
+    vol = _phantom(g1, 24)
+    ref = {id(g): _run(g, vol=vol).reshape(-1) for g in (g1, g2, g3)}
+    for g in (g1, g1, g2, g3, g1):
+        assert np.array_equal(dd.DD3Proj_gpu(g, vol).reshape(-1), ref[id(g)])
+    assert L.ctlab_plan_cache_size() == 3
+    y = np.random.default_rng(25).random(g2.nprj()).astype(np.float32)
+    assert np.array_equal(dd.DD3Back_gpu(g2, y).reshape(-1), _run(g2, prj=y).reshape(-1))
+    prev = L.ctlab_plan_cache_set_limit(1)
+    assert prev == 16 and L.ctlab_plan_cache_size() == 1
+    assert np.array_equal(dd.DD3Proj_gpu(g3, vol).reshape(-1), ref[id(g3)])
+    L.ctlab_plan_cache_set_limit(prev)
+    L.ctlab_plan_cache_clear()
+    assert L.ctlab_plan_cache_size() == 0
