@@ -2,6 +2,7 @@
 // images), activations and gradients in HBM, forward, forward+backward, Adam.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <vector>
 
@@ -12,15 +13,16 @@
 
 namespace ctlab {
 __global__ void k_conv32(const __bf16*, const __bf16*, const float*, const __bf16*, __bf16*, __bf16*, int, int,
-                         int, int);
-__global__ void k_conv_first(const float*, const float*, const float*, __bf16*, __bf16*, int, int, int);
+                         int, int, int);
+__global__ void k_conv_first(const float*, const float*, const float*, __bf16*, __bf16*, int, int, int, int);
 __global__ void k_conv_last(const __bf16*, const float*, const float*, const float*, float*, const float*, float*,
                             float, double*, int, int, int);
-__global__ void k_dgrad_last(const float*, const float*, const __bf16*, __bf16*, __bf16*, int, int, int);
+__global__ void k_dgrad_last(const float*, const float*, const __bf16*, __bf16*, __bf16*, int, int, int, int);
 __global__ void k_wgrad32(const __bf16*, const __bf16*, float*, int, int, int, long);
+__global__ void k_wgrad32_sweep(const __bf16*, const __bf16*, float*, int, int, int);
 __global__ void k_wgrad_first(const __bf16*, const float*, float*, int, int, int, long);
 __global__ void k_wgrad_last(const float*, const __bf16*, float*, int, int, int, long);
-__global__ void k_bias_grad_partial(const __bf16*, long, long, float*);
+__global__ void k_bias_grad_partial(const __bf16*, long, long, float*, int);
 __global__ void k_sum_partial(const float*, long, long, float*);
 __global__ void k_reduce_chunks(const float*, int, int, int, float*);
 __global__ void k_flip_transpose(const __bf16*, __bf16*);
@@ -42,6 +44,28 @@ constexpr long kNParams = 27 * CH + CH + 3 * (kW32 + CH) + 27 * CH + 1;
 constexpr long kWgradChunk = 8192;     // voxels per wgrad partial (multiple of 128: 4 wavefronts x runs of 32)
 constexpr long kVecChunk = 512;        // voxels per VALU wgrad partial
 constexpr long kRedTmp = 2048 * 64;    // reduce_chunks: groups x outputs <= (2048 / ceil(n/64)) * n
+constexpr long kSweepOut = kW32 + CH;  // k_wgrad32_sweep partial: a layer's W and b gradients
+
+// k_wgrad32_sweep reads the run-blocked copies (runs of 32 z) through one
+// buffer resource each (64 * nvox bytes) whose offsets, with up to two
+// invalid-marks of 1.5 * 2^30 added, must stay below 2^32 and past the
+// resource: ZN % 32 == 0 and 64 * nvox <= 2^30 (16.8M voxels, 512 x 512 x 64).
+// Other volumes keep channel-major copies and take k_wgrad32.
+bool sweep_ok(int XN, int YN, int ZN)
+{
+    return ZN % 32 == 0 && 64L * XN * YN * ZN <= (1L << 30);
+}
+
+// blocks of k_wgrad32_sweep: ~256 x-steps each, at most 512 (two 4-wavefront
+// blocks per CU at <= 256 VGPRs), a multiple of the 8 XCDs.  Fewer, longer
+// blocks also mean fewer 110 KB partials to add up (256 x 256 x 32: 256
+// blocks, 137 us + 10 us of reduction; 512 blocks, 140 + 20).
+long sweep_blocks(int XN, int YN, int ZN)
+{
+    const long steps = (long)YN * (ZN / 32) * XN;
+    const long nb = std::min(512L, std::max(8L, steps / 256));
+    return (nb + 7) / 8 * 8;
+}
 }  // namespace
 
 struct ctlab_cnn {
@@ -62,6 +86,7 @@ struct ctlab_cnn {
     float* rtmp = nullptr;              // reduce_chunks group partials (kRedTmp floats)
     double* lpart = nullptr;
     long nchunks = 0, nvchunks = 0;
+    long nsweep = 0;                    // k_wgrad32_sweep blocks (0: k_wgrad32 path)
     long adam_t = 0;
 };
 
@@ -102,16 +127,17 @@ int forward(ctlab_cnn* N, const float* x, float* out, const float* target, hipSt
 {
     const int XN = N->XN, YN = N->YN, ZN = N->ZN;
     const bool keep_cm = N->training;
+    const int rb = N->nsweep > 0;       // run-blocked channel-major copies
     // layer 1
     hipLaunchKernelGGL(k_conv_first, dim3(conv32_grid(XN, YN, ZN)), dim3(256), 0, s, x, N->wr, N->p + kOffW[0] + kLenW[0],
-                       N->a_cl[0], keep_cm ? N->a_cm[0] : (__bf16*)nullptr, XN, YN, ZN);
+                       N->a_cl[0], keep_cm ? N->a_cm[0] : (__bf16*)nullptr, XN, YN, ZN, rb);
     // layers 2..4 (inference ping-pongs between a_cl[0] and a_cl[1])
     for (int l = 1; l <= 3; ++l) {
         const __bf16* in = N->training ? N->a_cl[l - 1] : N->a_cl[(l - 1) & 1];
         __bf16* ocl = N->training ? N->a_cl[l] : N->a_cl[l & 1];
         __bf16* ocm = N->training ? N->a_cm[l] : nullptr;
         hipLaunchKernelGGL(k_conv32, dim3(conv32_grid(XN, YN, ZN)), dim3(256), 0, s, in, N->wb + (l - 1) * kW32,
-                           N->p + kOffW[l] + kLenW[l], (const __bf16*)nullptr, ocl, ocm, XN, YN, ZN, 1);
+                           N->p + kOffW[l] + kLenW[l], (const __bf16*)nullptr, ocl, ocm, XN, YN, ZN, 1, rb);
     }
     const __bf16* a4 = N->training ? N->a_cl[3] : N->a_cl[1];
     hipLaunchKernelGGL(k_conv_last, dim3((unsigned)conv_last_tiles(XN, YN, ZN)), dim3(256), 0, s, a4, N->wr + 27 * CH,
@@ -141,6 +167,7 @@ int backward(ctlab_cnn* N, const float* x, hipStream_t s)
 {
     const int XN = N->XN, YN = N->YN, ZN = N->ZN;
     const long n = N->nvox;
+    const int rb = N->nsweep > 0;       // run-blocked channel-major copies
     float* G = N->g;
     // layer 5
     hipLaunchKernelGGL(k_wgrad_last, dim3((unsigned)N->nvchunks), dim3(256), 0, s, N->g5, N->a_cl[3], N->part, XN, YN,
@@ -149,19 +176,26 @@ int backward(ctlab_cnn* N, const float* x, hipStream_t s)
     hipLaunchKernelGGL(k_sum_partial, dim3((unsigned)N->nvchunks), dim3(256), 0, s, N->g5, n, kVecChunk, N->part);
     reduce_chunks(N, N->part, (int)N->nvchunks, 1, G + kOffW[4] + kLenW[4], s);
     hipLaunchKernelGGL(k_dgrad_last, dim3(blocks(n, 256)), dim3(256), 0, s, N->g5, N->wr + 27 * CH, N->a_cl[3],
-                       N->g_cl[0], N->g_cm[0], XN, YN, ZN);
+                       N->g_cl[0], N->g_cm[0], XN, YN, ZN, rb);
     int cur = 0;   // g_*[cur] holds dL/dz of layer l+1 (l = 3, 2, 1 below: layers 4, 3, 2)
     for (int l = 3; l >= 1; --l) {
         // weight / bias gradients of layer l+1 (index l): input a_{l} = a_cl[l-1]
-        hipLaunchKernelGGL(k_wgrad32, dim3((unsigned)N->nchunks, 9), dim3(256), 0, s, N->g_cm[cur], N->a_cm[l - 1],
-                           N->part, XN, YN, ZN, kWgradChunk);
-        reduce_chunks(N, N->part, (int)N->nchunks, (int)kW32, G + kOffW[l], s);
-        hipLaunchKernelGGL(k_bias_grad_partial, dim3((unsigned)N->nvchunks), dim3(256), 0, s, N->g_cm[cur], n,
-                           kVecChunk, N->part);
-        reduce_chunks(N, N->part, (int)N->nvchunks, CH, G + kOffW[l] + kLenW[l], s);
+        if (N->nsweep) {
+            // W and b of the layer in one pass (b follows W in the flat vector)
+            hipLaunchKernelGGL(k_wgrad32_sweep, dim3((unsigned)N->nsweep), dim3(256), 0, s, N->g_cm[cur],
+                               N->a_cm[l - 1], N->part, XN, YN, ZN);
+            reduce_chunks(N, N->part, (int)N->nsweep, (int)kSweepOut, G + kOffW[l], s);
+        } else {
+            hipLaunchKernelGGL(k_wgrad32, dim3((unsigned)N->nchunks, 9), dim3(256), 0, s, N->g_cm[cur],
+                               N->a_cm[l - 1], N->part, XN, YN, ZN, kWgradChunk);
+            reduce_chunks(N, N->part, (int)N->nchunks, (int)kW32, G + kOffW[l], s);
+            hipLaunchKernelGGL(k_bias_grad_partial, dim3((unsigned)N->nvchunks), dim3(256), 0, s, N->g_cm[cur], n,
+                               kVecChunk, N->part, 0);
+            reduce_chunks(N, N->part, (int)N->nvchunks, CH, G + kOffW[l] + kLenW[l], s);
+        }
         // input gradient through layer l+1, masked by the ReLU of layer l
         hipLaunchKernelGGL(k_conv32, dim3(conv32_grid(XN, YN, ZN)), dim3(256), 0, s, N->g_cl[cur], N->wt + (l - 1) * kW32,
-                           (const float*)nullptr, N->a_cl[l - 1], N->g_cl[cur ^ 1], N->g_cm[cur ^ 1], XN, YN, ZN, 0);
+                           (const float*)nullptr, N->a_cl[l - 1], N->g_cl[cur ^ 1], N->g_cm[cur ^ 1], XN, YN, ZN, 0, rb);
         cur ^= 1;
     }
     // layer 1
@@ -169,7 +203,7 @@ int backward(ctlab_cnn* N, const float* x, hipStream_t s)
                        ZN, kVecChunk);
     reduce_chunks(N, N->part, (int)N->nvchunks, 27 * CH, G + kOffW[0], s);
     hipLaunchKernelGGL(k_bias_grad_partial, dim3((unsigned)N->nvchunks), dim3(256), 0, s, N->g_cm[cur], n, kVecChunk,
-                       N->part);
+                       N->part, rb);
     reduce_chunks(N, N->part, (int)N->nvchunks, CH, G + kOffW[0] + kLenW[0], s);
     return chk(hipGetLastError(), "cnn backward");
 }
@@ -205,6 +239,7 @@ ctlab_cnn* ctlab_cnn_create(int XN, int YN, int ZN, int training, int gpunum)
     const size_t A = (size_t)N->nvox * CH * sizeof(__bf16);
     N->nchunks = (N->nvox + kWgradChunk - 1) / kWgradChunk;
     N->nvchunks = (N->nvox + kVecChunk - 1) / kVecChunk;
+    N->nsweep = training && sweep_ok(XN, YN, ZN) ? sweep_blocks(XN, YN, ZN) : 0;
     bool ok = !chk(hipMalloc(&N->p, P), "hipMalloc") && !chk(hipMalloc(&N->wr, 2 * 27 * CH * sizeof(float)), "hipMalloc") &&
               !chk(hipMalloc(&N->wb, 3 * kW32 * sizeof(__bf16)), "hipMalloc") &&
               !chk(hipMalloc(&N->wt, 3 * kW32 * sizeof(__bf16)), "hipMalloc") &&
@@ -217,7 +252,8 @@ ctlab_cnn* ctlab_cnn_create(int XN, int YN, int ZN, int training, int gpunum)
         for (int i = 0; ok && i < 4; ++i) ok = !chk(hipMalloc(&N->a_cm[i], A), "hipMalloc(activations)");
         for (int i = 0; ok && i < 2; ++i)
             ok = !chk(hipMalloc(&N->g_cl[i], A), "hipMalloc(gradients)") && !chk(hipMalloc(&N->g_cm[i], A), "hipMalloc(gradients)");
-        const size_t part = std::max((size_t)N->nchunks * kW32, (size_t)N->nvchunks * 27 * CH) * sizeof(float);
+        const size_t part = std::max({(size_t)N->nchunks * kW32, (size_t)N->nvchunks * 27 * CH,
+                                      (size_t)N->nsweep * kSweepOut}) * sizeof(float);
         ok = ok && !chk(hipMalloc(&N->g, P), "hipMalloc") && !chk(hipMalloc(&N->m1, P), "hipMalloc") &&
              !chk(hipMalloc(&N->m2, P), "hipMalloc") && !chk(hipMalloc(&N->part, part), "hipMalloc") &&
              !chk(hipMalloc(&N->rtmp, kRedTmp * sizeof(float)), "hipMalloc") &&

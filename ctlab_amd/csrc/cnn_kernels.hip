@@ -6,6 +6,9 @@
 // Layouts (voxel v = (y*XN + x)*ZN + z, the volume's (Z,X,Y) order):
 //   channels-last  act_cl[v*32 + c]   (MFMA A-operand: 8 channels of a voxel = 16 B)
 //   channel-major  act_cm[c*nvox + v] (wgrad operands: 8 voxels of a channel = 16 B)
+//   run-blocked    act_rb[(v/32)*1024 + c*32 + v%32]  (ZN % 32 == 0: the same
+//                  operands, with the 32 channels of a run of 32 voxels in one 2 KB
+//                  block, so a wavefront's operand load is one contiguous KB)
 //   weights        w[t][co][ci], tap t = (dy+1)*9 + (dx+1)*3 + (dz+1)
 // Hidden 32 -> 32 layers are implicit GEMMs on v_mfma_f32_16x16x32_bf16:
 // one MFMA per (16 voxels, 16 output channels, tap) with K = the 32 input
@@ -43,6 +46,12 @@ __device__ __forceinline__ bool tap_ok(int t, int y, int x, int z, int YN, int X
 
 __device__ __forceinline__ float bf(const __bf16 v) { return (float)v; }
 
+// element (channel c, voxel v) of a channel-major (rb = 0) or run-blocked (rb = 1) copy
+__device__ __forceinline__ long cm_index(int c, long v, long nvox, int rb)
+{
+    return rb ? ((v >> 5) << 10) + ((long)c << 5) + (v & 31) : (long)c * nvox + v;
+}
+
 // ---------------------------------------------------------------------------
 // 32 -> 32 convolution, MFMA.  Block = 4 wavefronts, output tile = 8 x-columns
 // x 32 z of one y row (256 voxels; each wavefront two columns, 4 blocks of 16
@@ -69,7 +78,7 @@ constexpr int kBrick = kBX * kBZ;                   // 340 voxels
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4)))
 k_conv32(const __bf16* __restrict__ in, const __bf16* __restrict__ w, const float* __restrict__ bias,
          const __bf16* __restrict__ mask, __bf16* __restrict__ out_cl, __bf16* __restrict__ out_cm,
-         int XN, int YN, int ZN, int relu)
+         int XN, int YN, int ZN, int relu, int rb)
 {
     __shared__ bf16x8 sW[9 * CH * CH / 8];          // 18,432 B: the 9 taps of one dy plane
     __shared__ bf16x8 sA[kBrick * CH / 8];          // 21,760 B: the input brick of one plane
@@ -166,11 +175,11 @@ k_conv32(const __bf16* __restrict__ in, const __bf16* __restrict__ w, const floa
             }
             if (out_cm) {
                 if (zb + 3 < ZN && (ZN & 3) == 0) {     // v0 is a multiple of 4: 8-byte aligned vector store
-                    typedef __bf16 bf16x4 __attribute__((ext_vector_type(4)));
-                    *(bf16x4*)(out_cm + (long)co * nvox + v0) = (bf16x4){o[0], o[1], o[2], o[3]};
+                    typedef __bf16 bf16x4 __attribute__((ext_vector_type(4)));    // (one run when rb)
+                    *(bf16x4*)(out_cm + cm_index(co, v0, nvox, rb)) = (bf16x4){o[0], o[1], o[2], o[3]};
                 } else {
                     for (int i = 0; i < 4; ++i)
-                        if (zb + i < ZN) out_cm[(long)co * nvox + v0 + i] = o[i];
+                        if (zb + i < ZN) out_cm[cm_index(co, v0 + i, nvox, rb)] = o[i];
                 }
             }
         }
@@ -186,7 +195,7 @@ k_conv32(const __bf16* __restrict__ in, const __bf16* __restrict__ w, const floa
 // ---------------------------------------------------------------------------
 __global__ void __launch_bounds__(256)
 k_conv_first(const float* __restrict__ x, const float* __restrict__ w1, const float* __restrict__ b1,
-             __bf16* __restrict__ out_cl, __bf16* __restrict__ out_cm, int XN, int YN, int ZN)
+             __bf16* __restrict__ out_cl, __bf16* __restrict__ out_cm, int XN, int YN, int ZN, int rb)
 {
     __shared__ float sW[27 * CH + CH];
     __shared__ float sX[3 * kBrick];                        // 4,080 B
@@ -230,7 +239,7 @@ k_conv_first(const float* __restrict__ x, const float* __restrict__ w1, const fl
 #pragma unroll
     for (int c = 0; c < CH; ++c) {
         o[c] = (__bf16)fmaxf(acc[c], 0.0f);
-        if (out_cm) out_cm[(long)c * nvox + v] = o[c];
+        if (out_cm) out_cm[cm_index(c, v, nvox, rb)] = o[c];
     }
 #pragma unroll
     for (int q = 0; q < CH / 8; ++q)
@@ -347,7 +356,7 @@ k_conv_last(const __bf16* __restrict__ in, const float* __restrict__ w5, const f
 // g4[v][ci] = (a4[v][ci] > 0) * sum_t w5[t][ci] * g5[v - off(t)]
 __global__ void __launch_bounds__(256)
 k_dgrad_last(const float* __restrict__ g5, const float* __restrict__ w5, const __bf16* __restrict__ a4,
-             __bf16* __restrict__ g_cl, __bf16* __restrict__ g_cm, int XN, int YN, int ZN)
+             __bf16* __restrict__ g_cl, __bf16* __restrict__ g_cm, int XN, int YN, int ZN, int rb)
 {
     __shared__ float sW[27 * CH];
     for (int i = threadIdx.x; i < 27 * CH; i += 256) sW[i] = w5[i];
@@ -378,7 +387,7 @@ k_dgrad_last(const float* __restrict__ g5, const float* __restrict__ w5, const _
         for (int e = 0; e < 8; ++e) {
             const int c = 8 * q + e;
             o[c] = (__bf16)((float)mv[e] > 0.0f ? acc[c] : 0.0f);
-            g_cm[(long)c * nvox + v] = o[c];
+            g_cm[cm_index(c, v, nvox, rb)] = o[c];
         }
     }
 #pragma unroll
@@ -544,6 +553,159 @@ k_wgrad32(const __bf16* __restrict__ g_cm, const __bf16* __restrict__ a_cm, floa
     }
 }
 
+// ---------------------------------------------------------------------------
+// Weight and bias gradients of a 32 -> 32 layer on MFMA, swept along x
+// (ZN % 32 == 0; g and a as run-blocked copies):
+//   dW[t][co][ci] = sum_v g[v][co] a[v + off(t)][ci],   db[co] = sum_v g[v][co]
+// With z' = z + dz the z shift of tap t moves onto g:
+//   dW[t][co][ci] = sum_v' g[v' - dz][co] a[v' + off_xy(t)][ci]
+// (g and a read as 0 outside the volume), so each (dy, dx) column of a enters
+// its MFMAs unshifted and only g needs the dz = +-1 runs (alignbyte of its
+// 16-byte load and two neighbouring dwords).  Wavefront w = (cb, nb) of a
+// block owns the 16 x 16 block co in [16cb, 16cb+16), ci in [16nb, 16nb+16)
+// of all 27 taps (27 accumulators) and walks runs of 32 z (the K of one MFMA)
+// along x, holding the a columns at x-1, x, x+1 of rows y-1, y, y+1 in a ring
+// of kSwR slots: per step it loads g at x + kSwD and the three a columns at
+// x + 1 + kSwD (kSwD steps ahead) and issues 27 MFMAs and one for the bias
+// (B = 1): 6 loads per 28 MFMAs, each load one contiguous KB of a run block.
+// (Channel-major rows 2 * nvox bytes apart made each load touch 16 lines and
+// pages: 255 us instead of 137 us at 256 x 256 x 32.)
+// Work = the x-steps of all lines (y, z-run), dealt to blocks as contiguous
+// ranges, the ranges of one XCD contiguous (neighbouring lines read the same
+// rows of a through that XCD's L2).  Reads go through buffer resources whose
+// out-of-range offset reads 0: that supplies the zeros outside the volume and
+// past the end of a segment (steps past it multiply g = 0), so the unrolled
+// steps carry no branches.
+// Partials: part[block][27*32*32 + 32] = the layer's W gradient, then b.
+// ---------------------------------------------------------------------------
+constexpr int kSwD = 4;                 // prefetch distance (steps)
+constexpr int kSwR = kSwD + 3;          // ring slots (= steps per unrolled group)
+constexpr unsigned kBias = 0x60000000u; // added to an offset: past every resource (reads 0)
+constexpr int kBufW3 = 0x00020000;      // buffer-resource word 3: raw access on gfx9
+
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2)))
+k_wgrad32_sweep(const __bf16* __restrict__ g_rb, const __bf16* __restrict__ a_rb, float* __restrict__ part,
+                int XN, int YN, int ZN)
+{
+    const long nvox = (long)XN * YN * ZN;
+    const int nr = ZN >> 5;
+    const long T = (long)YN * nr * XN;             // x-steps of all lines
+    const long NB = gridDim.x;                     // a multiple of 8
+    const long j = (long)(blockIdx.x & 7) * (NB >> 3) + (blockIdx.x >> 3);
+    const long s0 = j * T / NB, s1 = (j + 1) * T / NB;
+
+    const int lane = threadIdx.x & 63;
+    const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);   // uniform: scalar resources
+    const int m = lane & 15, kq = lane >> 4;
+    const int cb = wv >> 1, nb = wv & 1;
+    // one resource per tensor, nbytes = 64 * nvox <= 2^30 (host check): an
+    // offset holding one or two kBias stays below 2^32 and past the resource
+    const unsigned nbytes = (unsigned)(64 * nvox);
+    const __amdgpu_buffer_rsrc_t rg = __builtin_amdgcn_make_buffer_rsrc((void*)g_rb, (short)0, (int)nbytes, kBufW3);
+    const __amdgpu_buffer_rsrc_t ra = __builtin_amdgcn_make_buffer_rsrc((void*)a_rb, (short)0, (int)nbytes, kBufW3);
+    // this lane's 8 elements in a run block: channel 16cb + m (g) / 16nb + m (a), z 8kq..8kq+7
+    const unsigned lane_g = (unsigned)(((cb * 16 + m) * 32 + 8 * kq) * 2);
+    const unsigned lane_a = (unsigned)(((nb * 16 + m) * 32 + 8 * kq) * 2);
+
+    f32x4 acc[27], accb = (f32x4){0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int t = 0; t < 27; ++t) acc[t] = (f32x4){0.f, 0.f, 0.f, 0.f};
+    const bf16x8 ones = __builtin_bit_cast(bf16x8, (u32x4){0x3F803F80u, 0x3F803F80u, 0x3F803F80u, 0x3F803F80u});
+
+    for (long s = s0; s < s1;) {                   // one segment of a line per pass (wave-uniform)
+        const long ln = s / XN;
+        const int x0 = (int)(s - ln * XN);
+        const int xend = (int)min((long)XN, x0 + (s1 - s));
+        const int y = (int)(ln / nr), r = (int)(ln - (long)y * nr);
+        // byte offsets: lane part + uniform part (scalar); an invalid row,
+        // column or step adds kBias, which takes the sum past the resource.
+        // Run (y, x, r) starts at byte ((y * XN + x) * nr + r) * 2048.
+        unsigned rowb[3];
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            const int yy = y + d - 1;
+            rowb[d] = (unsigned)yy < (unsigned)YN ? (unsigned)(((long)yy * XN * nr + r) * 2048) : kBias;
+        }
+        const unsigned gbase = (unsigned)(((long)y * XN * nr + r) * 2048);
+        const unsigned cstep = (unsigned)nr * 2048u;
+        const int cmax = min(XN, xend + 1);               // columns x0-1 .. xend are read
+        // g at z - 2, z - 1 and z + 8, z + 9 of the lane's 8: inside its own
+        // channel row, in the previous / next run of the column (+-1024
+        // elements), or outside the column (kBias)
+        const unsigned padj = kq > 0 ? (unsigned)-4 : r > 0 ? (unsigned)(-(1024 - 30) * 2) : kBias;
+        const unsigned nadj = kq < 3 ? 16u : r < nr - 1 ? (unsigned)((1024 - 24) * 2) : kBias;
+
+        u32x4 col[kSwR][3];                        // a columns, slot (c - x0 + 1) % kSwR
+        u32x4 gb[kSwR];                            // g at x, slot (x - x0) % kSwR
+        uint32_t gp[kSwR], gn[kSwR];               // its neighbours z-1 (elements -2, -1), z+8 (8, 9)
+        auto load_col = [&](u32x4 (&c3)[3], int c) {
+            const unsigned cofs = (unsigned)c < (unsigned)cmax ? (unsigned)c * cstep : kBias;
+#pragma unroll
+            for (int d = 0; d < 3; ++d)
+                c3[d] = __builtin_bit_cast(u32x4, __builtin_amdgcn_raw_buffer_load_b128(
+                                                      ra, lane_a + (rowb[d] + cofs), 0, 0));
+        };
+        auto load_g = [&](u32x4& b, uint32_t& p, uint32_t& n, int x) {
+            const unsigned off = lane_g + (x < xend ? gbase + (unsigned)x * cstep : kBias);
+            b = __builtin_bit_cast(u32x4, __builtin_amdgcn_raw_buffer_load_b128(rg, off, 0, 0));
+            p = __builtin_amdgcn_raw_buffer_load_b32(rg, off + padj, 0, 0);
+            n = __builtin_amdgcn_raw_buffer_load_b32(rg, off + nadj, 0, 0);
+        };
+#pragma unroll
+        for (int i = 0; i < kSwD + 2; ++i) load_col(col[i], x0 - 1 + i);
+#pragma unroll
+        for (int i = 0; i < kSwD; ++i) load_g(gb[i], gp[i], gn[i], x0 + i);
+
+        for (int base = x0; base < xend; base += kSwR) {
+#pragma unroll
+            for (int k = 0; k < kSwR; ++k) {
+                const int x = base + k;
+                // keep each step's operand math after its loads: hoisted next to
+                // them, it would wait on the loads just issued
+                __builtin_amdgcn_sched_barrier(0);
+                load_col(col[(k + kSwR - 1) % kSwR], x + 1 + kSwD);
+                load_g(gb[(k + kSwD) % kSwR], gp[(k + kSwD) % kSwR], gn[(k + kSwD) % kSwR], x + kSwD);
+                const u32x4 b = gb[k];
+                // dz = +1 pairs a at z' with g at z' - 1: elements -1 .. 6 of the run
+                u32x4 lo;
+                lo[0] = __builtin_amdgcn_alignbyte(b[0], gp[k], 2);
+                lo[1] = __builtin_amdgcn_alignbyte(b[1], b[0], 2);
+                lo[2] = __builtin_amdgcn_alignbyte(b[2], b[1], 2);
+                lo[3] = __builtin_amdgcn_alignbyte(b[3], b[2], 2);
+                // dz = -1 pairs a at z' with g at z' + 1: elements 1 .. 8
+                u32x4 hi;
+                hi[0] = __builtin_amdgcn_alignbyte(b[1], b[0], 2);
+                hi[1] = __builtin_amdgcn_alignbyte(b[2], b[1], 2);
+                hi[2] = __builtin_amdgcn_alignbyte(b[3], b[2], 2);
+                hi[3] = __builtin_amdgcn_alignbyte(gn[k], b[3], 2);
+                const bf16x8 Gm = as_bf(hi), G0 = as_bf(b), Gp = as_bf(lo);
+#pragma unroll
+                for (int dy = 0; dy < 3; ++dy)
+#pragma unroll
+                    for (int dx = 0; dx < 3; ++dx) {
+                        const bf16x8 B = as_bf(col[(k + dx) % kSwR][dy]);
+                        const int t = dy * 9 + dx * 3;
+                        acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(Gm, B, acc[t], 0, 0, 0);
+                        acc[t + 1] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(G0, B, acc[t + 1], 0, 0, 0);
+                        acc[t + 2] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(Gp, B, acc[t + 2], 0, 0, 0);
+                    }
+                accb = __builtin_amdgcn_mfma_f32_16x16x32_bf16(G0, ones, accb, 0, 0, 0);
+            }
+        }
+        s += xend - x0;
+    }
+    // D: rows co = 16cb + 4kq + i, column ci = 16nb + m
+    float* P = part + j * (27 * CH * CH + CH);
+#pragma unroll
+    for (int t = 0; t < 27; ++t)
+#pragma unroll
+        for (int i = 0; i < 4; ++i) P[((long)t * CH + cb * 16 + 4 * kq + i) * CH + nb * 16 + m] = acc[t][i];
+    if (nb == 0 && m == 0) {
+#pragma unroll
+        for (int i = 0; i < 4; ++i) P[27 * CH * CH + cb * 16 + 4 * kq + i] = accb[i];
+    }
+}
+
 // Weight gradients of the thin layers (VALU), one block per chunk of voxels,
 // partials part[chunk][27*32].  Thread r = tid % 128 < 108 owns tap t = r / 4
 // and the 8 channels 8*(r % 4)..+7 of it: per voxel one 16-byte load of those
@@ -678,13 +840,13 @@ k_wgrad_last(const float* __restrict__ g5, const __bf16* __restrict__ a_cl, floa
 // chunks are then summed in a fixed order (k_reduce_chunks), so the result
 // does not depend on scheduling.
 __global__ void __launch_bounds__(256)
-k_bias_grad_partial(const __bf16* __restrict__ g_cm, long nvox, long chunk, float* __restrict__ part)
+k_bias_grad_partial(const __bf16* __restrict__ g_cm, long nvox, long chunk, float* __restrict__ part, int rb)
 {
     const long c0 = blockIdx.x * chunk, c1 = min(c0 + chunk, nvox);
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     for (int c = wv; c < CH; c += 4) {
         float s = 0.0f;
-        for (long v = c0 + lane; v < c1; v += 64) s += bf(g_cm[(long)c * nvox + v]);
+        for (long v = c0 + lane; v < c1; v += 64) s += bf(g_cm[cm_index(c, v, nvox, rb)]);
         for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
         if (lane == 0) part[(long)blockIdx.x * CH + c] = s;
     }
