@@ -61,27 +61,64 @@ __device__ __forceinline__ long cm_index(int c, long v, long nvox, int rb)
 // Writes channels-last and (optionally) channel-major copies.
 // Per dy plane, the tile's input brick (10 columns x 34 z of row y+dy, all 32
 // channels, zero outside the volume) and the plane's 9 taps of weights are
-// staged in LDS (21.8 + 18.4 KB: 4 blocks per CU); the 9 taps then read their
+// staged in LDS (22.5 + 18.4 KB: 4 blocks per CU); the 9 taps then read their
 // MFMA operands from LDS, so each input voxel is fetched from L2 about 1.3
 // times per plane instead of 9.  The next plane's brick is loaded into
 // registers while the current plane's MFMAs run (a plane's 72 MFMAs per
 // wavefront are shorter than one round of global loads; fetching after the
 // barrier left the MFMAs idle: 1.52 -> 1.16 ms per layer at 512x512x64).
-// Prefetching the weights as well needs 168 registers and ran slower.
-// Registers are held to 128 (4 wavefronts per SIMD, what the LDS allows):
-// 136 registers and 3 wavefronts ran 14 % slower.  Tiles are dealt to the 8 XCDs in contiguous
-// ranges (y rows of one range share their input rows in that XCD's L2).
+// Prefetching the weights into registers as well spills at 128 registers
+// (2x slower).  Registers are held to 128 (4 wavefronts per SIMD, what the LDS
+// allows): 136 registers and 3 wavefronts ran 14 % slower.  What bounds it
+// (37-39 % MFMA busy at 512x512x64, no LDS bank conflicts): the MFMAs of a
+// tap wait on their LDS operands (ds_read, s_waitcnt lgkmcnt(0), 1-2 MFMAs),
+// the 128-register budget leaving no room to read a tap ahead.  Tiles are
+// dealt to the 8 XCDs in contiguous ranges of the band order below.
 // ---------------------------------------------------------------------------
 constexpr int kBX = kTX + 2, kBZ = kTZ + 2;         // input brick of one plane
 constexpr int kBrick = kBX * kBZ;                   // 340 voxels
+
+// LDS tiles are stored as chunk planes: 16-byte chunk q (channels 8q..8q+7)
+// of row p (a voxel of the brick, or an output channel of a tap's weights) at
+// slot q * stride + p, so lane (m, kq) of an MFMA operand reads chunk kq of
+// row base + m and a tap's shift stays a constant offset.  Measured on gfx950
+// (ds_read_b128, 64 lanes as here): a read takes 4 LDS cycles when the plane
+// stride is a multiple of 256 bytes, and 8 (2-way bank conflicts) with 64-byte
+// rows (the layout [row][chunk]) or with a plane stride of 5440, 4160 or 4224
+// bytes.  Planes are filled in the same (row, chunk) lane pattern they are
+// read in: staging index i holds row 16 * (i / 64) + i % 16, chunk (i / 16) % 4.
+constexpr int kBrickP = 352;                        // brick plane stride: 5,632 B
+constexpr int kXBand = 16;                          // x-tiles per band of the tile order
+
+// Tile order of k_conv32: bands of kXBand x-tiles (128 columns); within a band
+// y, then x, then z.  An XCD's concurrent blocks then cover a few y rows of
+// one band, whose input rows (y-1 .. y+1) stay in its L2, instead of whole
+// rows (512 x 512 x 64: three 2 MB input rows per output row); 4 % faster.
+// Bands past the last full one are narrower (ntx % kXBand tiles).
+__device__ __forceinline__ void conv32_band_tile(long r, int ntx, int YN, int& xt, int& y)
+{
+    const int nfull = ntx / kXBand;
+    const long fullt = (long)nfull * kXBand * YN;
+    if (r < fullt) {
+        const long bb = r / ((long)kXBand * YN), rr = r - bb * kXBand * YN;
+        y = (int)(rr / kXBand);
+        xt = (int)(bb * kXBand + rr % kXBand);
+    } else {
+        const int wl = ntx - nfull * kXBand;
+        const long rr = r - fullt;
+        y = (int)(rr / wl);
+        xt = nfull * kXBand + (int)(rr % wl);
+    }
+}
+constexpr int kWP = 9 * CH;                         // weight plane stride: 4,608 B
 
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4)))
 k_conv32(const __bf16* __restrict__ in, const __bf16* __restrict__ w, const float* __restrict__ bias,
          const __bf16* __restrict__ mask, __bf16* __restrict__ out_cl, __bf16* __restrict__ out_cm,
          int XN, int YN, int ZN, int relu, int rb)
 {
-    __shared__ bf16x8 sW[9 * CH * CH / 8];          // 18,432 B: the 9 taps of one dy plane
-    __shared__ bf16x8 sA[kBrick * CH / 8];          // 21,760 B: the input brick of one plane
+    __shared__ bf16x8 sW[4 * kWP];                  // 18,432 B: the 9 taps of one dy plane
+    __shared__ bf16x8 sA[4 * kBrickP];              // 22,528 B: the input brick of one plane
     const long ntiles = conv32_tiles(XN, YN, ZN);
     // contiguous tile ranges per XCD (blocks are dealt to XCDs round-robin); the
     // grid is 8 * ceil(ntiles / 8), so (xcd, j) covers [0, 8 * per) once
@@ -90,9 +127,8 @@ k_conv32(const __bf16* __restrict__ in, const __bf16* __restrict__ w, const floa
     if (tile >= ntiles) return;                     // whole block: before any barrier
     const int ntz = (ZN + kTZ - 1) / kTZ, ntx = (XN + kTX - 1) / kTX;
     const int zt = (int)(tile % ntz);
-    const long r = tile / ntz;
-    const int xt = (int)(r % ntx);
-    const int y = (int)(r / ntx);
+    int xt, y;
+    conv32_band_tile(tile / ntz, ntx, YN, xt, y);
     const int x0 = xt * kTX, z0 = zt * kTZ;
 
     const int lane = threadIdx.x & 63;
@@ -109,18 +145,21 @@ k_conv32(const __bf16* __restrict__ in, const __bf16* __restrict__ w, const floa
     for (int b = 0; b < 4; ++b) acc[b][0] = acc[b][1] = (f32x4){0.f, 0.f, 0.f, 0.f};
 
     const bf16x8 zero8 = (bf16x8){0, 0, 0, 0, 0, 0, 0, 0};
-    constexpr int kNA = (kBrick * CH / 8 + 255) / 256;
+    constexpr int kNA = (kBrickP * CH / 8 + 255) / 256;
     bf16x8 pa[kNA];
+    // staging index i -> (row, chunk), the lane pattern of the operand reads
+    auto srow = [](int i) { return ((i >> 6) << 4) | (i & 15); };
+    auto schunk = [](int i) { return (i >> 4) & 3; };
     auto fetch = [&](int plane) {
         const int yy = y + plane - 1;
         const bool row_in = (unsigned)yy < (unsigned)YN;
 #pragma unroll
         for (int j = 0; j < kNA; ++j) {
             const int i = threadIdx.x + 256 * j;
-            const int p = i >> 2, q = i & 3;
+            const int p = srow(i), q = schunk(i);
             const int gx = x0 - 1 + p / kBZ, gz = z0 - 1 + p % kBZ;
             pa[j] = zero8;
-            if (i < kBrick * CH / 8 && row_in && (unsigned)gx < (unsigned)XN && (unsigned)gz < (unsigned)ZN)
+            if (p < kBrick && row_in && (unsigned)gx < (unsigned)XN && (unsigned)gz < (unsigned)ZN)
                 pa[j] = ((const bf16x8*)in)[(((long)yy * XN + gx) * ZN + gz) * (CH / 8) + q];
         }
     };
@@ -128,22 +167,22 @@ k_conv32(const __bf16* __restrict__ in, const __bf16* __restrict__ w, const floa
     for (int plane = 0; plane < 3; ++plane) {
         __syncthreads();   // the previous plane's operands are no longer read
         for (int i = threadIdx.x; i < 9 * CH * CH / 8; i += 256)
-            sW[i] = ((const bf16x8*)w)[plane * 9 * CH * CH / 8 + i];
+            sW[schunk(i) * kWP + srow(i)] = ((const bf16x8*)w)[plane * 9 * CH * CH / 8 + srow(i) * 4 + schunk(i)];
 #pragma unroll
         for (int j = 0; j < kNA; ++j) {
             const int i = threadIdx.x + 256 * j;
-            if (i < kBrick * CH / 8) sA[i] = pa[j];
+            if (srow(i) < kBrick) sA[schunk(i) * kBrickP + srow(i)] = pa[j];
         }
         __syncthreads();
         if (plane < 2) fetch(plane + 1);   // in flight during this plane's MFMAs
 #pragma unroll
         for (int tt = 0; tt < 9; ++tt) {
             const int shift = (tt / 3 - 1) * kBZ + (tt % 3 - 1);      // (dx, dz) in the brick
-            const bf16x8 b0 = sW[((tt * CH) + m) * (CH / 8) + kq];
-            const bf16x8 b1 = sW[((tt * CH) + 16 + m) * (CH / 8) + kq];
+            const bf16x8 b0 = sW[kq * kWP + tt * CH + m];
+            const bf16x8 b1 = sW[kq * kWP + tt * CH + 16 + m];
 #pragma unroll
             for (int b = 0; b < 4; ++b) {
-                const bf16x8 a = sA[(pb[b] + shift) * (CH / 8) + kq];
+                const bf16x8 a = sA[kq * kBrickP + pb[b] + shift];
                 acc[b][0] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b0, acc[b][0], 0, 0, 0);
                 acc[b][1] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b1, acc[b][1], 0, 0, 0);
             }
