@@ -303,8 +303,9 @@ __host__ __device__ inline int zmode_of(const DevGeom& g) { return !g.cone ? kZF
 // neighbour: its upper boundary (z = 64) mostly projects off the detector,
 // where the prefix is the column total (above) or 0 (below), and is
 // interpolated only when it lands on the detector; the DPP shift leaves lane 63
-// its own fill value.  top_total: the view's top boundary is above the detector
-// for every slab and column (decided on the host).  The differences follow the
+// its own fill value.  The column total is G[DNV], which lane 63 holds as Pu
+// (its v index clamps to DNV - 1).  top_total: the view's top boundary is
+// above the detector for every slab and column (decided on the host).  The differences follow the
 // detector row axis; the caller orients them along z (sign of the row pitch).
 template <int ZM, typename T>
 __device__ __forceinline__ T column_delta(const DevGeom& g, T* G, T Pu, double esdz, double lzb, double zoff,
@@ -319,12 +320,12 @@ __device__ __forceinline__ T column_delta(const DevGeom& g, T* G, T Pu, double e
         if constexpr (ZM == kZCone) {
             return shift_down1(Q) - Q;
         } else {
-            T top;   // G[DNV] is the column total
+            T top;   // only lane 63's value is used
             if (top_total) {
-                top = G[DNV];
+                top = Pu;
             } else {
                 const double ntop = (64.0 - beta0) * inv_sj * esdz - zoff;
-                top = ntop >= (double)DNV ? G[DNV] : ntop <= 0.0 ? T(0) : lds_interp(G, ntop, DNV);
+                top = ntop >= (double)DNV ? Pu : ntop <= 0.0 ? T(0) : lds_interp(G, ntop, DNV);
             }
             return shift_down1_fill(Q, top) - Q;
         }
