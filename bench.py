@@ -888,6 +888,9 @@ def main():
     dominant = "proj" if np.mean(tp) >= np.mean(tb) else "back"
     ach = ach_p if dominant == "proj" else ach_b
     traffic = _traffic()
+    traffic_b = ((traffic or {}).get("by_tiles", {}).get("%d,%d" % plan.tiles(), {})
+                 .get(dominant + "_bytes_per_call"))
+    t_dom = float(np.mean(tp if dominant == "proj" else tb))
     out = {
         "metric": METRIC,
         "value": 2.0 * vv / t_pair,
@@ -909,11 +912,18 @@ def main():
         "autotune_ms": tune,
         "roofline": {"bound": "hbm", "achieved": ach / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
                      "frac": ach / HBM_PEAK,
-                     "traffic": ((traffic or {}).get("by_tiles", {}).get("%d,%d" % plan.tiles(), {})
-                                 .get(dominant + "_bytes_per_call")),
+                     "traffic": traffic_b,
                      "kernel": "dd_proj_walk (+zprefix)" if dominant == "proj"
                                else "dd_back_walk (+prjprefix, back_add)",
-                     "pair_frac": (2 * bytes_op / t_pair) / HBM_PEAK},
+                     "pair_frac": (2 * bytes_op / t_pair) / HBM_PEAK,
+                     # `achieved` is the streaming model's bytes (4 B x PN x (voxels + cells)) over the
+                     # operator's time, i.e. an effective bandwidth; the counters' bytes over the same time
+                     # are the HBM bandwidth the operator really draws
+                     "achieved_is": "streaming-model bytes / time (effective bandwidth)",
+                     "measured_gbps": traffic_b / t_dom / 1e9 if traffic_b else None,
+                     "measured_frac": traffic_b / t_dom / HBM_PEAK if traffic_b else None,
+                     "limit": "VALU issue (SQ counters: 72-82 % VALU busy at 8 wavefronts per SIMD), "
+                              "not HBM bandwidth"},
         "ms_proj": float(np.mean(tp)) * 1e3,
         "ms_back": float(np.mean(tb)) * 1e3,
         "wall_s": wall,
