@@ -189,30 +189,64 @@ k_conv32(const __bf16* __restrict__ in, const __bf16* __restrict__ w, const floa
         }
     }
 
-    // D fragment: lane holds rows 4*kq + i (voxels z), column m (output channel)
+    // Epilogue through LDS.  The D fragments (lane: 4 voxels of one output
+    // channel) go, with bias and ReLU, into this wavefront's [voxel][channel]
+    // patch of the free weight tile; each lane then writes 16-byte chunks (8
+    // channels of one voxel) of the channels-last output, reading the ReLU
+    // mask of the input-gradient pass in the same chunks.  Writing the
+    // fragments straight out took 64 two-byte stores per lane, ~20 % of the
+    // kernel at 512x512x64.
     const long nvox = (long)XN * YN * ZN;
+    __syncthreads();                                // every wavefront is done with sW / sA
+    __bf16* sO = (__bf16*)sW + wv * 64 * CH;        // 64 voxels x 32 channels (4 KB) of this wavefront
 #pragma unroll
     for (int b = 0; b < 4; ++b) {
-        const int x = x0 + 2 * wv + (b >> 1);
-        const int zb = z0 + 16 * (b & 1) + 4 * kq;
-        if (x >= XN) continue;
-        const long v0 = ((long)y * XN + x) * ZN + zb;
 #pragma unroll
         for (int nb = 0; nb < 2; ++nb) {
             const int co = nb * 16 + m;
             const float bco = bias ? bias[co] : 0.0f;
-            __bf16 o[4];
 #pragma unroll
             for (int i = 0; i < 4; ++i) {
-                const long v = v0 + i;
-                const bool ok = zb + i < ZN;
                 float rr = acc[b][nb][i] + bco;
                 if (relu) rr = fmaxf(rr, 0.0f);
-                if (mask && ok && !(bf(mask[v * CH + co]) > 0.0f)) rr = 0.0f;
-                o[i] = (__bf16)rr;
-                if (ok) out_cl[v * CH + co] = o[i];
+                sO[(16 * b + 4 * kq + i) * CH + co] = (__bf16)rr;
             }
-            if (out_cm) {
+        }
+    }
+    // LDS executes one wavefront's accesses in issue order: the chunks below
+    // see the values this wavefront's lanes wrote above
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const int c = lane + 64 * k;                // chunk: voxel c / 4, channels 8 * (c % 4) ..
+        const int vb = c >> 2, q = c & 3;
+        const int x = x0 + 2 * wv + (vb >> 5);
+        const int z = z0 + (vb & 31);
+        if (x >= XN || z >= ZN) continue;
+        const long v = ((long)y * XN + x) * ZN + z;
+        bf16x8 o = *(const bf16x8*)(sO + vb * CH + 8 * q);
+        if (mask) {
+            const bf16x8 mk = *(const bf16x8*)(mask + v * CH + 8 * q);
+#pragma unroll
+            for (int e = 0; e < 8; ++e)
+                if (!(bf(mk[e]) > 0.0f)) o[e] = (__bf16)0.0f;
+            if (out_cm) *(bf16x8*)(sO + vb * CH + 8 * q) = o;   // the masked values for the copy below
+        }
+        *(bf16x8*)(out_cl + v * CH + 8 * q) = o;
+    }
+    if (out_cm) {
+        // channel-major (or run-blocked) copy: lane = (channel m of block nb, 4 voxels)
+#pragma unroll
+        for (int b = 0; b < 4; ++b) {
+            const int x = x0 + 2 * wv + (b >> 1);
+            const int zb = z0 + 16 * (b & 1) + 4 * kq;
+            if (x >= XN) continue;
+            const long v0 = ((long)y * XN + x) * ZN + zb;
+#pragma unroll
+            for (int nb = 0; nb < 2; ++nb) {
+                const int co = nb * 16 + m;
+                __bf16 o[4];
+#pragma unroll
+                for (int i = 0; i < 4; ++i) o[i] = sO[(16 * b + 4 * kq + i) * CH + co];
                 if (zb + 3 < ZN && (ZN & 3) == 0) {     // v0 is a multiple of 4: 8-byte aligned vector store
                     typedef __bf16 bf16x4 __attribute__((ext_vector_type(4)));    // (one run when rb)
                     *(bf16x4*)(out_cm + cm_index(co, v0, nvox, rb)) = (bf16x4){o[0], o[1], o[2], o[3]};
