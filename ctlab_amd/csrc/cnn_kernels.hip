@@ -15,8 +15,9 @@
 // channels.  The input gradient runs the same kernel with flipped, transposed
 // weights and the ReLU mask fused into its epilogue; the weight gradient runs
 // MFMAs over K = voxels on the channel-major copies.  The 32 -> 1 output
-// layer runs on MFMA with the (dy, dx) taps as the N dimension; the 1 -> 32
-// first layer and the thin gradient kernels run on the VALU.
+// layer runs on MFMA with the (dy, dx) taps as the N dimension, and so does
+// the 1 -> 32 first layer (its FP32 input split into three exact BF16 parts);
+// the thin gradient kernels run on the VALU.
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
@@ -260,18 +261,25 @@ k_conv32(const __bf16* __restrict__ in, const __bf16* __restrict__ w, const floa
 }
 
 // ---------------------------------------------------------------------------
-// first layer 1 -> 32 (VALU): out = relu(conv(x; w1) + b1); w1 fp32 [27][32].
-// Same tiles as k_conv32 (8 x-columns x 32 z of one y row, thread = voxel):
-// the tile's input brick (3 rows x 10 columns x 34 z of x, zero outside the
-// volume) is staged in LDS with coalesced loads, so the 27 taps read their
-// operand from LDS instead of waiting on one global load each.
+// first layer 1 -> 32 on MFMA: out = relu(conv(x; w1) + b1); w1 FP32 [27][32]
+// holding BF16-rounded values.  Per v_mfma_f32_16x16x32_bf16: M = 16 voxels,
+// N = 16 output channels, K = the 27 taps (padded to 32).  The FP32 input
+// enters as three BF16 parts, x = hi + mid + lo with hi = bf16(x), mid =
+// bf16(x - hi), lo = x - hi - mid; each part holds 8 significant bits, so the
+// split is exact, every product with a BF16 weight is exact, and the FP32
+// accumulation gives the FP32 sum of the former VALU kernel up to the order of
+// the additions (that kernel took ~1,100 VALU instructions per voxel, 0.62 ms
+// at 512x512x64).  Same tiles as k_conv32 (8 x-columns x 32 z of one y row;
+// wavefront w: columns 2w, 2w+1, 4 blocks of 16 voxels); the tile's input
+// brick (3 rows x 10 columns x 34 z, zero outside the volume) sits in LDS, and
+// the output goes through LDS in 16-byte chunks as in k_conv32.
 // ---------------------------------------------------------------------------
 __global__ void __launch_bounds__(256)
 k_conv_first(const float* __restrict__ x, const float* __restrict__ w1, const float* __restrict__ b1,
              __bf16* __restrict__ out_cl, __bf16* __restrict__ out_cm, int XN, int YN, int ZN, int rb)
 {
-    __shared__ float sW[27 * CH + CH];
     __shared__ float sX[3 * kBrick];                        // 4,080 B
+    __shared__ bf16x8 sO8[4 * 64 * CH / 8];                 // 16 KB: each wavefront's 64 voxels x 32 channels
     const long ntiles = conv32_tiles(XN, YN, ZN);
     const long per = (long)gridDim.x >> 3;
     const long tile = (long)(blockIdx.x & 7) * per + (blockIdx.x >> 3);
@@ -281,8 +289,6 @@ k_conv_first(const float* __restrict__ x, const float* __restrict__ w1, const fl
     const long r = tile / ntz;
     const int x0 = (int)(r % ntx) * kTX, y = (int)(r / ntx), z0 = zt * kTZ;
 
-    for (int i = threadIdx.x; i < 27 * CH; i += 256) sW[i] = w1[i];
-    for (int i = threadIdx.x; i < CH; i += 256) sW[27 * CH + i] = b1[i];
     for (int i = threadIdx.x; i < 3 * kBrick; i += 256) {
         const int pl = i / kBrick, p = i - pl * kBrick;
         const int gy = y + pl - 1, gx = x0 - 1 + p / kBZ, gz = z0 - 1 + p % kBZ;
@@ -291,33 +297,94 @@ k_conv_first(const float* __restrict__ x, const float* __restrict__ w1, const fl
             v = x[((long)gy * XN + gx) * ZN + gz];
         sX[i] = v;
     }
+    const int lane = threadIdx.x & 63;
+    const int wv = threadIdx.x >> 6;
+    const int m = lane & 15, kq = lane >> 4;
+    // B fragments (lane: taps 8kq .. 8kq+7 of output channel 16nb + m; taps 27..31 are 0)
+    bf16x8 B[2];
+#pragma unroll
+    for (int nb = 0; nb < 2; ++nb)
+#pragma unroll
+        for (int e = 0; e < 8; ++e) {
+            const int t = 8 * kq + e;
+            B[nb][e] = (__bf16)(t < 27 ? w1[t * CH + nb * 16 + m] : 0.0f);
+        }
     __syncthreads();
-    const int col = threadIdx.x >> 5, zz = threadIdx.x & 31;
-    const int xx = x0 + col, z = z0 + zz;
-    if (xx >= XN || z >= ZN) return;                        // no barrier follows
+
+    f32x4 acc[4][2];
+#pragma unroll
+    for (int b = 0; b < 4; ++b) acc[b][0] = acc[b][1] = (f32x4){0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int b = 0; b < 4; ++b) {
+        // block b: column 2wv + (b >> 1), z 16(b & 1) + m; its centre voxel in the brick of row y
+        const int ctr = kBrick + (2 * wv + (b >> 1) + 1) * kBZ + 16 * (b & 1) + m + 1;
+        bf16x8 hi, mid, lo;
+#pragma unroll
+        for (int e = 0; e < 8; ++e) {
+            const int t = 8 * kq + e;
+            const int dy = t / 9 - 1, dx = (t / 3) % 3 - 1, dz = t % 3 - 1;
+            const float a = t < 27 ? sX[ctr + dy * kBrick + dx * kBZ + dz] : 0.0f;
+            hi[e] = (__bf16)a;
+            const float r1 = a - (float)hi[e];
+            mid[e] = (__bf16)r1;
+            lo[e] = (__bf16)(r1 - (float)mid[e]);
+        }
+#pragma unroll
+        for (int nb = 0; nb < 2; ++nb) {
+            acc[b][nb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(hi, B[nb], acc[b][nb], 0, 0, 0);
+            acc[b][nb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(mid, B[nb], acc[b][nb], 0, 0, 0);
+            acc[b][nb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(lo, B[nb], acc[b][nb], 0, 0, 0);
+        }
+    }
+
+    // epilogue: fragments (lane: voxels 4kq + i of block b, channel 16nb + m) with bias and
+    // ReLU into this wavefront's [voxel][channel] patch, then 16-byte chunks out
     const long nvox = (long)XN * YN * ZN;
-    const long v = ((long)y * XN + xx) * ZN + z;
-    const int base = kBrick + (col + 1) * kBZ + zz + 1;     // centre of the brick, row y
-    float acc[CH];
+    __bf16* sO = (__bf16*)sO8 + wv * 64 * CH;
 #pragma unroll
-    for (int c = 0; c < CH; ++c) acc[c] = sW[27 * CH + c];
-#pragma unroll 1
-    for (int t = 0; t < 27; ++t) {
-        const int dy = t / 9 - 1, dx = (t / 3) % 3 - 1, dz = t % 3 - 1;
-        const float a = sX[base + dy * kBrick + dx * kBZ + dz];
+    for (int b = 0; b < 4; ++b)
 #pragma unroll
-        for (int c = 0; c < CH; ++c) acc[c] += sW[t * CH + c] * a;
+        for (int nb = 0; nb < 2; ++nb) {
+            const int co = nb * 16 + m;
+            const float bco = b1[co];
+#pragma unroll
+            for (int i = 0; i < 4; ++i)
+                sO[(16 * b + 4 * kq + i) * CH + co] = (__bf16)fmaxf(acc[b][nb][i] + bco, 0.0f);
+        }
+    // LDS executes one wavefront's accesses in issue order
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const int c = lane + 64 * k;                        // chunk: voxel c / 4, channels 8 * (c % 4) ..
+        const int vb = c >> 2, q = c & 3;
+        const int xx = x0 + 2 * wv + (vb >> 5);
+        const int z = z0 + (vb & 31);
+        if (xx >= XN || z >= ZN) continue;
+        const long v = ((long)y * XN + xx) * ZN + z;
+        *(bf16x8*)(out_cl + v * CH + 8 * q) = *(const bf16x8*)(sO + vb * CH + 8 * q);
     }
-    __bf16 o[CH];
+    if (out_cm) {
 #pragma unroll
-    for (int c = 0; c < CH; ++c) {
-        o[c] = (__bf16)fmaxf(acc[c], 0.0f);
-        if (out_cm) out_cm[cm_index(c, v, nvox, rb)] = o[c];
+        for (int b = 0; b < 4; ++b) {
+            const int xx = x0 + 2 * wv + (b >> 1);
+            const int zb = z0 + 16 * (b & 1) + 4 * kq;
+            if (xx >= XN) continue;
+            const long v0 = ((long)y * XN + xx) * ZN + zb;
+#pragma unroll
+            for (int nb = 0; nb < 2; ++nb) {
+                const int co = nb * 16 + m;
+                __bf16 o[4];
+#pragma unroll
+                for (int i = 0; i < 4; ++i) o[i] = sO[(16 * b + 4 * kq + i) * CH + co];
+                if (zb + 3 < ZN && (ZN & 3) == 0) {
+                    typedef __bf16 bf16x4 __attribute__((ext_vector_type(4)));
+                    *(bf16x4*)(out_cm + cm_index(co, v0, nvox, rb)) = (bf16x4){o[0], o[1], o[2], o[3]};
+                } else {
+                    for (int i = 0; i < 4; ++i)
+                        if (zb + i < ZN) out_cm[cm_index(co, v0 + i, nvox, rb)] = o[i];
+                }
+            }
+        }
     }
-#pragma unroll
-    for (int q = 0; q < CH / 8; ++q)
-        *(bf16x8*)(out_cl + v * CH + 8 * q) = (bf16x8){o[8 * q], o[8 * q + 1], o[8 * q + 2], o[8 * q + 3],
-                                                        o[8 * q + 4], o[8 * q + 5], o[8 * q + 6], o[8 * q + 7]};
 }
 
 // ---------------------------------------------------------------------------
