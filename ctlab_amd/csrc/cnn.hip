@@ -14,6 +14,7 @@
 namespace ctlab {
 __global__ void k_conv32(const __bf16*, const __bf16*, const float*, const __bf16*, __bf16*, __bf16*, int, int,
                          int, int, int);
+template <bool kCM>
 __global__ void k_conv_first(const float*, const float*, const float*, __bf16*, __bf16*, int, int, int, int);
 __global__ void k_conv_last(const __bf16*, const float*, const float*, const float*, float*, const float*, float*,
                             float, double*, int, int, int);
@@ -129,8 +130,12 @@ int forward(ctlab_cnn* N, const float* x, float* out, const float* target, hipSt
     const bool keep_cm = N->training;
     const int rb = N->nsweep > 0;       // run-blocked channel-major copies
     // layer 1
-    hipLaunchKernelGGL(k_conv_first, dim3(conv32_grid(XN, YN, ZN)), dim3(256), 0, s, x, N->wr, N->p + kOffW[0] + kLenW[0],
-                       N->a_cl[0], keep_cm ? N->a_cm[0] : (__bf16*)nullptr, XN, YN, ZN, rb);
+    if (keep_cm)
+        hipLaunchKernelGGL(k_conv_first<true>, dim3(conv_first_grid(XN, YN, ZN)), dim3(256), 0, s, x, N->wr,
+                           N->p + kOffW[0] + kLenW[0], N->a_cl[0], N->a_cm[0], XN, YN, ZN, rb);
+    else
+        hipLaunchKernelGGL(k_conv_first<false>, dim3(conv_first_grid(XN, YN, ZN)), dim3(256), 0, s, x, N->wr,
+                           N->p + kOffW[0] + kLenW[0], N->a_cl[0], (__bf16*)nullptr, XN, YN, ZN, rb);
     // layers 2..4 (inference ping-pongs between a_cl[0] and a_cl[1])
     for (int l = 1; l <= 3; ++l) {
         const __bf16* in = N->training ? N->a_cl[l - 1] : N->a_cl[(l - 1) & 1];
@@ -140,7 +145,7 @@ int forward(ctlab_cnn* N, const float* x, float* out, const float* target, hipSt
                            N->p + kOffW[l] + kLenW[l], (const __bf16*)nullptr, ocl, ocm, XN, YN, ZN, 1, rb);
     }
     const __bf16* a4 = N->training ? N->a_cl[3] : N->a_cl[1];
-    hipLaunchKernelGGL(k_conv_last, dim3((unsigned)conv_last_tiles(XN, YN, ZN)), dim3(256), 0, s, a4, N->wr + 27 * CH,
+    hipLaunchKernelGGL(k_conv_last, dim3(conv_last_grid(XN, YN, ZN)), dim3(256), 0, s, a4, N->wr + 27 * CH,
                        N->p + kOffW[4] + kLenW[4], x, out, target, N->g5, (float)(1.0 / (double)N->nvox),
                        target ? N->lpart : (double*)nullptr, XN, YN, ZN);
     return chk(hipGetLastError(), "cnn forward");
@@ -308,7 +313,7 @@ double ctlab_cnn_loss_grad(ctlab_cnn* N, const float* d_x, const float* d_target
     DeviceGuard dg(N->device);
     hipStream_t s = (hipStream_t)stream;
     if (forward(N, d_x, d_out ? d_out : N->out, d_target, s) || backward(N, d_x, s)) return NAN;
-    const long nb = conv_last_tiles(N->XN, N->YN, N->ZN);       // one partial per k_conv_last block
+    const long nb = conv_last_tiles(N->XN, N->YN, N->ZN);       // one partial per k_conv_last tile
     std::vector<double> lp(nb);
     if (chk(hipMemcpyAsync(lp.data(), N->lpart, nb * sizeof(double), hipMemcpyDeviceToHost, s), "D2H")) return NAN;
     if (h_grad && chk(hipMemcpyAsync(h_grad, N->g, kNParams * sizeof(float), hipMemcpyDeviceToHost, s), "D2H"))

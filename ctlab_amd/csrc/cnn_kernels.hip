@@ -269,34 +269,61 @@ k_conv32(const __bf16* __restrict__ in, const __bf16* __restrict__ w, const floa
 // split is exact, every product with a BF16 weight is exact, and the FP32
 // accumulation gives the FP32 sum of the former VALU kernel up to the order of
 // the additions (that kernel took ~1,100 VALU instructions per voxel, 0.62 ms
-// at 512x512x64).  Same tiles as k_conv32 (8 x-columns x 32 z of one y row;
-// wavefront w: columns 2w, 2w+1, 4 blocks of 16 voxels); the tile's input
-// brick (3 rows x 10 columns x 34 z, zero outside the volume) sits in LDS, and
-// the output goes through LDS in 16-byte chunks as in k_conv32.
+// at 512x512x64).  A block computes the k_conv32 tile (8 x-columns x 32 z;
+// wavefront w: columns 2w, 2w+1, 4 blocks of 16 voxels) for kFY consecutive y
+// rows: the input planes (10 columns x 34 z of one y row, zero outside the
+// volume) sit in a ring of three LDS slots, and plane y + 2 is read into
+// registers while row y is computed, so each input plane is fetched once per
+// block and the weight fragments once per kFY rows (one row per block with all
+// three planes staged before the MFMAs: 0.57 ms at 512x512x64, the load latency
+// exposed).  The output goes through LDS in 16-byte chunks as in k_conv32.
 // ---------------------------------------------------------------------------
+// kCM: also write the channel-major copy out_cm (training); a separate instance,
+// since its address arithmetic, held across the row loop, costs 41 registers.
+template <bool kCM>
 __global__ void __launch_bounds__(256)
 k_conv_first(const float* __restrict__ x, const float* __restrict__ w1, const float* __restrict__ b1,
              __bf16* __restrict__ out_cl, __bf16* __restrict__ out_cm, int XN, int YN, int ZN, int rb)
 {
-    __shared__ float sX[3 * kBrick];                        // 4,080 B
+    __shared__ float sX[3 * kBrick];                        // 4,080 B: planes y-1, y, y+1 (ring)
     __shared__ bf16x8 sO8[4 * 64 * CH / 8];                 // 16 KB: each wavefront's 64 voxels x 32 channels
-    const long ntiles = conv32_tiles(XN, YN, ZN);
+    const long ntiles = conv_first_tiles(XN, YN, ZN);
     const long per = (long)gridDim.x >> 3;
     const long tile = (long)(blockIdx.x & 7) * per + (blockIdx.x >> 3);
     if (tile >= ntiles) return;                             // whole block: before any barrier
     const int ntz = (ZN + kTZ - 1) / kTZ, ntx = (XN + kTX - 1) / kTX;
     const int zt = (int)(tile % ntz);
     const long r = tile / ntz;
-    const int x0 = (int)(r % ntx) * kTX, y = (int)(r / ntx), z0 = zt * kTZ;
+    const int x0 = (int)(r % ntx) * kTX, z0 = zt * kTZ;
+    const int y0 = (int)(r / ntx) * kFY, ny = min(kFY, YN - y0);
 
-    for (int i = threadIdx.x; i < 3 * kBrick; i += 256) {
-        const int pl = i / kBrick, p = i - pl * kBrick;
-        const int gy = y + pl - 1, gx = x0 - 1 + p / kBZ, gz = z0 - 1 + p % kBZ;
+    // plane element p (column p / kBZ, z p % kBZ of the brick) of row yy, 0 outside the volume
+    auto element = [&](int yy, int p) {
+        const int gx = x0 - 1 + p / kBZ, gz = z0 - 1 + p % kBZ;
         float v = 0.0f;
-        if ((unsigned)gy < (unsigned)YN && (unsigned)gx < (unsigned)XN && (unsigned)gz < (unsigned)ZN)
-            v = x[((long)gy * XN + gx) * ZN + gz];
-        sX[i] = v;
+        if (p < kBrick && (unsigned)yy < (unsigned)YN && (unsigned)gx < (unsigned)XN && (unsigned)gz < (unsigned)ZN)
+            v = x[((long)yy * XN + gx) * ZN + gz];
+        return v;
+    };
+    constexpr int kNP = (kBrick + 255) / 256;               // plane elements per thread
+    // slot of plane yy: (yy - y0 + 1) % 3; planes y0 - 1 and y0 go straight in
+#pragma unroll
+    for (int j = 0; j < kNP; ++j) {
+        const int p = threadIdx.x + 256 * j;
+        if (p < kBrick) {
+            sX[p] = element(y0 - 1, p);
+            sX[kBrick + p] = element(y0, p);
+        }
     }
+    // planes y0 + 1 and y0 + 2 in registers; row y then writes plane y + 1 to its slot and
+    // fetches plane y + 3, which has two rows' time to arrive
+    float pfA[kNP], pfB[kNP];
+#pragma unroll
+    for (int j = 0; j < kNP; ++j) {
+        pfA[j] = element(y0 + 1, threadIdx.x + 256 * j);
+        pfB[j] = element(y0 + 2, threadIdx.x + 256 * j);
+    }
+
     const int lane = threadIdx.x & 63;
     const int wv = threadIdx.x >> 6;
     const int m = lane & 15, kq = lane >> 4;
@@ -309,109 +336,140 @@ k_conv_first(const float* __restrict__ x, const float* __restrict__ w1, const fl
             const int t = 8 * kq + e;
             B[nb][e] = (__bf16)(t < 27 ? w1[t * CH + nb * 16 + m] : 0.0f);
         }
-    __syncthreads();
-
-    f32x4 acc[4][2];
-#pragma unroll
-    for (int b = 0; b < 4; ++b) acc[b][0] = acc[b][1] = (f32x4){0.f, 0.f, 0.f, 0.f};
-#pragma unroll
-    for (int b = 0; b < 4; ++b) {
-        // block b: column 2wv + (b >> 1), z 16(b & 1) + m; its centre voxel in the brick of row y
-        const int ctr = kBrick + (2 * wv + (b >> 1) + 1) * kBZ + 16 * (b & 1) + m + 1;
-        bf16x8 hi, mid, lo;
-#pragma unroll
-        for (int e = 0; e < 8; ++e) {
-            const int t = 8 * kq + e;
-            const int dy = t / 9 - 1, dx = (t / 3) % 3 - 1, dz = t % 3 - 1;
-            const float a = t < 27 ? sX[ctr + dy * kBrick + dx * kBZ + dz] : 0.0f;
-            hi[e] = (__bf16)a;
-            const float r1 = a - (float)hi[e];
-            mid[e] = (__bf16)r1;
-            lo[e] = (__bf16)(r1 - (float)mid[e]);
-        }
-#pragma unroll
-        for (int nb = 0; nb < 2; ++nb) {
-            acc[b][nb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(hi, B[nb], acc[b][nb], 0, 0, 0);
-            acc[b][nb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(mid, B[nb], acc[b][nb], 0, 0, 0);
-            acc[b][nb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(lo, B[nb], acc[b][nb], 0, 0, 0);
-        }
-    }
-
-    // epilogue: fragments (lane: voxels 4kq + i of block b, channel 16nb + m) with bias and
-    // ReLU into this wavefront's [voxel][channel] patch, then 16-byte chunks out
+    const float bias0 = b1[m], bias1 = b1[16 + m];
     const long nvox = (long)XN * YN * ZN;
     __bf16* sO = (__bf16*)sO8 + wv * 64 * CH;
+
+    auto row = [&](int s, float (&pf)[kNP]) {
+        const int y = y0 + s;
+        __syncthreads();                                    // row y - 1's gathers are done with its slot
 #pragma unroll
-    for (int b = 0; b < 4; ++b)
-#pragma unroll
-        for (int nb = 0; nb < 2; ++nb) {
-            const int co = nb * 16 + m;
-            const float bco = b1[co];
-#pragma unroll
-            for (int i = 0; i < 4; ++i)
-                sO[(16 * b + 4 * kq + i) * CH + co] = (__bf16)fmaxf(acc[b][nb][i] + bco, 0.0f);
+        for (int j = 0; j < kNP; ++j) {
+            const int p = threadIdx.x + 256 * j;
+            if (p < kBrick) sX[((s + 2) % 3) * kBrick + p] = pf[j];
         }
-    // LDS executes one wavefront's accesses in issue order
+        __syncthreads();
+        if (s + 2 < ny) {                                   // plane y + 3, in flight during rows y, y + 1
 #pragma unroll
-    for (int k = 0; k < 4; ++k) {
-        const int c = lane + 64 * k;                        // chunk: voxel c / 4, channels 8 * (c % 4) ..
-        const int vb = c >> 2, q = c & 3;
-        const int xx = x0 + 2 * wv + (vb >> 5);
-        const int z = z0 + (vb & 31);
-        if (xx >= XN || z >= ZN) continue;
-        const long v = ((long)y * XN + xx) * ZN + z;
-        *(bf16x8*)(out_cl + v * CH + 8 * q) = *(const bf16x8*)(sO + vb * CH + 8 * q);
-    }
-    if (out_cm) {
+            for (int j = 0; j < kNP; ++j) pf[j] = element(y + 3, threadIdx.x + 256 * j);
+        }
+        // lane's 8 taps: offset of tap 8kq + e from a voxel's centre (the centre in slot (s + 1) % 3)
+        int off[8];
+#pragma unroll
+        for (int e = 0; e < 8; ++e) {
+            const int t = min(8 * kq + e, 26);
+            const int dy = t / 9 - 1, dx = (t / 3) % 3 - 1, dz = t % 3 - 1;
+            off[e] = ((s + 1 + dy + 3) % 3) * kBrick + dx * kBZ + dz;
+        }
+
+        // block b: column 2wv + (b >> 1), z 16(b & 1) + m.  Its fragments (lane: voxels
+        // 4kq + i, channel 16nb + m) go with bias and ReLU into this wavefront's
+        // [voxel][channel] patch, then 16-byte chunks out.  LDS executes one wavefront's
+        // accesses in issue order, so the patch needs no barrier between rows.
 #pragma unroll
         for (int b = 0; b < 4; ++b) {
-            const int xx = x0 + 2 * wv + (b >> 1);
-            const int zb = z0 + 16 * (b & 1) + 4 * kq;
-            if (xx >= XN) continue;
-            const long v0 = ((long)y * XN + xx) * ZN + zb;
+            const int ctr = (2 * wv + (b >> 1) + 1) * kBZ + 16 * (b & 1) + m + 1;   // centre in the brick
+            bf16x8 hi, mid, lo;
+#pragma unroll
+            for (int e = 0; e < 8; ++e) {
+                const float a = 8 * kq + e < 27 ? sX[ctr + off[e]] : 0.0f;
+                hi[e] = (__bf16)a;
+                const float r1 = a - (float)hi[e];
+                mid[e] = (__bf16)r1;
+                lo[e] = (__bf16)(r1 - (float)mid[e]);
+            }
 #pragma unroll
             for (int nb = 0; nb < 2; ++nb) {
-                const int co = nb * 16 + m;
-                __bf16 o[4];
+                f32x4 acc = (f32x4){0.f, 0.f, 0.f, 0.f};
+                acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(hi, B[nb], acc, 0, 0, 0);
+                acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(mid, B[nb], acc, 0, 0, 0);
+                acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(lo, B[nb], acc, 0, 0, 0);
+                const float bco = nb ? bias1 : bias0;
 #pragma unroll
-                for (int i = 0; i < 4; ++i) o[i] = sO[(16 * b + 4 * kq + i) * CH + co];
-                if (zb + 3 < ZN && (ZN & 3) == 0) {
-                    typedef __bf16 bf16x4 __attribute__((ext_vector_type(4)));
-                    *(bf16x4*)(out_cm + cm_index(co, v0, nvox, rb)) = (bf16x4){o[0], o[1], o[2], o[3]};
-                } else {
-                    for (int i = 0; i < 4; ++i)
-                        if (zb + i < ZN) out_cm[cm_index(co, v0 + i, nvox, rb)] = o[i];
+                for (int i = 0; i < 4; ++i)
+                    sO[(16 * b + 4 * kq + i) * CH + nb * 16 + m] = (__bf16)fmaxf(acc[i] + bco, 0.0f);
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const int c = lane + 64 * k;                    // chunk: voxel c / 4, channels 8 * (c % 4) ..
+            const int vb = c >> 2, q = c & 3;
+            const int xx = x0 + 2 * wv + (vb >> 5);
+            const int z = z0 + (vb & 31);
+            if (xx >= XN || z >= ZN) continue;
+            const long v = ((long)y * XN + xx) * ZN + z;
+            *(bf16x8*)(out_cl + v * CH + 8 * q) = *(const bf16x8*)(sO + vb * CH + 8 * q);
+        }
+        if (kCM) {
+#pragma unroll
+            for (int b = 0; b < 4; ++b) {
+                const int xx = x0 + 2 * wv + (b >> 1);
+                const int zb = z0 + 16 * (b & 1) + 4 * kq;
+                if (xx >= XN) continue;
+                const long v0 = ((long)y * XN + xx) * ZN + zb;
+#pragma unroll
+                for (int nb = 0; nb < 2; ++nb) {
+                    const int co = nb * 16 + m;
+                    __bf16 o[4];
+#pragma unroll
+                    for (int i = 0; i < 4; ++i) o[i] = sO[(16 * b + 4 * kq + i) * CH + co];
+                    if (zb + 3 < ZN && (ZN & 3) == 0) {
+                        typedef __bf16 bf16x4 __attribute__((ext_vector_type(4)));
+                        *(bf16x4*)(out_cm + cm_index(co, v0, nvox, rb)) = (bf16x4){o[0], o[1], o[2], o[3]};
+                    } else {
+                        for (int i = 0; i < 4; ++i)
+                            if (zb + i < ZN) out_cm[cm_index(co, v0 + i, nvox, rb)] = o[i];
+                    }
                 }
             }
         }
+    };
+    for (int s = 0; s < ny; s += 2) {                       // two rows per trip: pfA, pfB stay registers
+        row(s, pfA);
+        if (s + 1 < ny) row(s + 1, pfB);
     }
 }
+template __global__ void k_conv_first<false>(const float*, const float*, const float*, __bf16*, __bf16*, int, int, int, int);
+template __global__ void k_conv_first<true>(const float*, const float*, const float*, __bf16*, __bf16*, int, int, int, int);
 
 // ---------------------------------------------------------------------------
 // last layer 32 -> 1, MFMA: out = x + conv(a4; w5) + b5; training forward
 // also writes g5 = (out - target) * gscale and the block's sum of squares.
-// Block = 4 wavefronts, output tile = 8 x-columns x 4 y-rows x 32 z.  Each of
-// the tile's 10 x 6 input columns (with the halo) and 16-voxel z block gives,
-// in one MFMA chain over dz (K = 32 channels each), its shares of the 9 output
-// columns it reaches: B column n = (dy, dx) holds the weights of tap
-// (dy, dx, dz), so D[z][n] is input column c's share of output column
-// c - (dy, dx).  Every share has its own LDS slot (n, output column, z), and
-// each output adds its 9 shares in a fixed order (deterministic).  w5 holds
-// the BF16-rounded weights as FP32, so the B operand is exact.
+// Block = 4 wavefronts, output tile = 8 x-columns x 32 z, for kLY consecutive
+// y rows.  Each input column of a row (the tile's 10 columns with the halo) and
+// 16-voxel z block gives, in one MFMA chain over dz (K = 32 channels each),
+// its shares of the 9 output columns it reaches: B column n = (dy, dx) holds
+// the weights of tap (dy, dx, dz), so D[z][n] is input column c's share of
+// output column c - (dy, dx), in output row (input row) - dy.  Every share has
+// its own LDS slot (row, n, output column, z), in a ring of three output rows;
+// after input row y + 1 row y is complete and each of its outputs adds its 9
+// shares in a fixed order (deterministic).  The block walks the input rows
+// y0 - 1 .. y0 + kLY once: a row (10 columns x 34 z x 32 channels, zero
+// outside the volume) is staged in LDS as k_conv32's brick, the next one read
+// into registers meanwhile, so each input voxel leaves HBM about once (tiles of
+// 8 x 4 output columns, each reading its own 10 x 6 input columns: 2.1 GB at
+// 512x512x64, 0.53 ms).  w5 holds the BF16-rounded weights as FP32, so the B
+// operand is exact.
 // ---------------------------------------------------------------------------
 __global__ void __launch_bounds__(256)
 k_conv_last(const __bf16* __restrict__ in, const float* __restrict__ w5, const float* __restrict__ b5,
             const float* __restrict__ x, float* __restrict__ out, const float* __restrict__ target,
             float* __restrict__ g5, float gscale, double* __restrict__ part, int XN, int YN, int ZN)
 {
-    constexpr int kCols = kLX * kLY;                        // 32 output columns
-    constexpr int kIX = kLX + 2, kIY = kLY + 2;             // 10 x 6 input columns
-    __shared__ f32x4 sD[9 * kCols * kTZ / 4];               // 36,864 B: shares (n, column, z)
+    constexpr int kIX = kLX + 2;                            // 10 input columns
+    constexpr int kRow = 9 * kLX * kTZ;                     // one output row's shares (n, column, z)
+    __shared__ bf16x8 sA[4 * kBrickP];                      // 22,528 B: the input row
+    __shared__ f32x4 sD[3 * kRow / 4];                      // 27,648 B: three output rows' shares
     __shared__ double sred[4];
+    const long ntiles = conv_last_tiles(XN, YN, ZN);
+    const long per = (long)gridDim.x >> 3;                  // contiguous tile ranges per XCD
+    const long tile = (long)(blockIdx.x & 7) * per + (blockIdx.x >> 3);
+    if (tile >= ntiles) return;                             // whole block: before any barrier
     const int ntz = (ZN + kTZ - 1) / kTZ, ntx = (XN + kLX - 1) / kLX;
-    const int zt = (int)(blockIdx.x % ntz);
-    const int r = (int)(blockIdx.x / ntz);
-    const int x0 = (r % ntx) * kLX, y0 = (r / ntx) * kLY, z0 = zt * kTZ;
+    const int zt = (int)(tile % ntz);
+    const long r = tile / ntz;
+    const int x0 = (int)(r % ntx) * kLX, y0 = (int)(r / ntx) * kLY, z0 = zt * kTZ;
+    const int ny = min(kLY, YN - y0);
 
     const int lane = threadIdx.x & 63;
     const int wv = threadIdx.x >> 6;
@@ -428,67 +486,81 @@ k_conv_last(const __bf16* __restrict__ in, const float* __restrict__ w5, const f
         }
     }
     const int nzb = ZN - z0 > 16 ? 2 : 1;                   // z blocks of 16 that hold voxels
-    const int nit = kIX * kIY * nzb;
-    // item it = (input column c, z block zb); a wavefront takes it = wv, wv + 4, ...
-    // and keeps the loads of the next two items in flight
-    auto load_item = [&](int it, bf16x8 (&a)[3]) {
-        const int c = it / nzb, zb = it - c * nzb;
-        const int iy = c / kIX, ix = c - iy * kIX;
-        const int gy = y0 - 1 + iy, gx = x0 - 1 + ix;
-        const bool ok = it < nit && (unsigned)gy < (unsigned)YN && (unsigned)gx < (unsigned)XN;
-        const long cbase = ((long)gy * XN + gx) * ZN;
+    const int nit = kIX * nzb;                              // items (input column, z block) per row
+
+    const bf16x8 zero8 = (bf16x8){0, 0, 0, 0, 0, 0, 0, 0};
+    constexpr int kNA = (kBrickP * CH / 8 + 255) / 256;
+    bf16x8 pa[kNA];
+    // staging index i -> (brick row, chunk), the lane pattern of the operand reads
+    auto srow = [](int i) { return ((i >> 6) << 4) | (i & 15); };
+    auto schunk = [](int i) { return (i >> 4) & 3; };
+    auto fetch = [&](int yy) {
+        const bool row_in = (unsigned)yy < (unsigned)YN;
 #pragma unroll
-        for (int dz = 0; dz < 3; ++dz) {
-            const int z = z0 + 16 * zb + m + dz - 1;
-            a[dz] = (bf16x8){0, 0, 0, 0, 0, 0, 0, 0};
-            if (ok && (unsigned)z < (unsigned)ZN) a[dz] = *(const bf16x8*)(in + (cbase + z) * CH + 8 * kq);
+        for (int j = 0; j < kNA; ++j) {
+            const int i = threadIdx.x + 256 * j;
+            const int p = srow(i), q = schunk(i);
+            const int gx = x0 - 1 + p / kBZ, gz = z0 - 1 + p % kBZ;
+            pa[j] = zero8;
+            if (p < kBrick && row_in && (unsigned)gx < (unsigned)XN && (unsigned)gz < (unsigned)ZN)
+                pa[j] = ((const bf16x8*)in)[(((long)yy * XN + gx) * ZN + gz) * (CH / 8) + q];
         }
     };
-    bf16x8 a0[3], a1[3], a2[3];
-    load_item(wv, a0);
-    load_item(wv + 4, a1);
-    for (int it = wv; it < nit; it += 4) {                  // wave-uniform trip count
-        load_item(it + 8, a2);
-        f32x4 acc = (f32x4){0.f, 0.f, 0.f, 0.f};
-#pragma unroll
-        for (int dz = 0; dz < 3; ++dz) acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a0[dz], B[dz], acc, 0, 0, 0);
-        // D: lane holds rows z = 16*zb + 4*kq + i of column n = m
-        const int c = it / nzb, zb = it - c * nzb;
-        const int iy = c / kIX, ix = c - iy * kIX;
-        const int oy = iy - m / 3, ox = ix - m % 3;             // output column c - (dy, dx)
-        if (m < 9 && (unsigned)oy < (unsigned)kLY && (unsigned)ox < (unsigned)kLX)
-            sD[((m * kCols + oy * kLX + ox) * kTZ + 16 * zb + 4 * kq) >> 2] = acc;
-#pragma unroll
-        for (int dz = 0; dz < 3; ++dz) {
-            a0[dz] = a1[dz];
-            a1[dz] = a2[dz];
-        }
-    }
-    __syncthreads();
-    const float* D = (const float*)sD;
     const float bias = b5[0];
     double r2 = 0.0;
-    for (int j = threadIdx.x; j < kCols * kTZ; j += 256) {
-        const int col = j / kTZ, zz = j - col * kTZ;
-        const int gy = y0 + col / kLX, gx = x0 + col % kLX, gz = z0 + zz;
-        if (gy >= YN || gx >= XN || gz >= ZN) continue;
-        float sum = bias;
+    fetch(y0 - 1);
+    // step g: input row gy = y0 - 1 + g gives its shares to output rows gy + 1, gy, gy - 1,
+    // then output row gy - 1 is complete
+    for (int g = 0; g < ny + 2; ++g) {
+        const int gy = y0 - 1 + g;
 #pragma unroll
-        for (int n = 0; n < 9; ++n) sum += D[(n * kCols + col) * kTZ + zz];
-        const long v = ((long)gy * XN + gx) * ZN + gz;
-        const float o = x[v] + sum;
-        out[v] = o;
-        if (target) {
-            const float rr = o - target[v];
-            g5[v] = rr * gscale;
-            r2 += (double)rr * rr;
+        for (int j = 0; j < kNA; ++j) {
+            const int i = threadIdx.x + 256 * j;
+            if (srow(i) < kBrick) sA[schunk(i) * kBrickP + srow(i)] = pa[j];
+        }
+        __syncthreads();   // row staged; the previous step's finalisation is done with its slot
+        if (g + 1 < ny + 2) fetch(gy + 1);                  // in flight during this row
+        for (int it = wv; it < nit; it += 4) {              // wave-uniform trip count
+            const int ix = it / nzb, zb = it - ix * nzb;
+            f32x4 acc = (f32x4){0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+            for (int dz = 0; dz < 3; ++dz) {
+                // rows z0 + 16 zb + m + dz - 1: brick z 16 zb + m + dz
+                const bf16x8 a = sA[kq * kBrickP + ix * kBZ + 16 * zb + m + dz];
+                acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, B[dz], acc, 0, 0, 0);
+            }
+            // D: lane holds rows z = 16*zb + 4*kq + i of column n = m (dy = m / 3, dx = m % 3):
+            // output row gy + 1 - dy, tile column ix - dx
+            const int oy = gy + 1 - m / 3 - y0, ox = ix - m % 3;
+            if (m < 9 && (unsigned)oy < (unsigned)ny && (unsigned)ox < (unsigned)kLX)
+                sD[(((oy + 3) % 3) * kRow + (m * kLX + ox) * kTZ + 16 * zb + 4 * kq) >> 2] = acc;
+        }
+        __syncthreads();   // shares of this row written; sA free for the next row
+        const int oy = gy - 1 - y0;                         // the completed output row
+        if (oy >= 0) {
+            const float* D = (const float*)sD + (oy % 3) * kRow;
+            const int col = threadIdx.x / kTZ, zz = threadIdx.x - col * kTZ;
+            const int gx = x0 + col, gz = z0 + zz;
+            if (gx < XN && gz < ZN) {
+                float sum = bias;
+#pragma unroll
+                for (int n = 0; n < 9; ++n) sum += D[(n * kLX + col) * kTZ + zz];
+                const long v = ((long)(y0 + oy) * XN + gx) * ZN + gz;
+                const float o = x[v] + sum;
+                out[v] = o;
+                if (target) {
+                    const float rr = o - target[v];
+                    g5[v] = rr * gscale;
+                    r2 += (double)rr * rr;
+                }
+            }
         }
     }
     if (part) {
         for (int sh = 32; sh > 0; sh >>= 1) r2 += __shfl_xor(r2, sh, 64);
         if (lane == 0) sred[wv] = r2;
         __syncthreads();
-        if (threadIdx.x == 0) part[blockIdx.x] = sred[0] + sred[1] + sred[2] + sred[3];
+        if (threadIdx.x == 0) part[tile] = sred[0] + sred[1] + sred[2] + sred[3];
     }
 }
 

@@ -17,11 +17,26 @@ inline unsigned conv32_grid(int XN, int YN, int ZN)
 {
     return (unsigned)((conv32_tiles(XN, YN, ZN) + 7) / 8 * 8);
 }
-// output layer (32 -> 1): 8 x-columns x 4 y-rows x 32 z per block
-constexpr int kLX = 8, kLY = 4;
+// first layer (1 -> 32): the same tile for kFY consecutive y rows per block
+constexpr int kFY = 8;
 
-inline long conv_last_tiles(int XN, int YN, int ZN)
+__host__ __device__ inline long conv_first_tiles(int XN, int YN, int ZN)
+{
+    return (long)((YN + kFY - 1) / kFY) * ((XN + kTX - 1) / kTX) * ((ZN + kTZ - 1) / kTZ);
+}
+inline unsigned conv_first_grid(int XN, int YN, int ZN)
+{
+    return (unsigned)((conv_first_tiles(XN, YN, ZN) + 7) / 8 * 8);
+}
+// output layer (32 -> 1): 8 x-columns x 32 z, for kLY consecutive y rows per block
+constexpr int kLX = 8, kLY = 16;
+
+__host__ __device__ inline long conv_last_tiles(int XN, int YN, int ZN)
 {
     return (long)((YN + kLY - 1) / kLY) * ((XN + kLX - 1) / kLX) * ((ZN + kTZ - 1) / kTZ);
+}
+inline unsigned conv_last_grid(int XN, int YN, int ZN)
+{
+    return (unsigned)((conv_last_tiles(XN, YN, ZN) + 7) / 8 * 8);
 }
 }  // namespace ctlab
