@@ -27,6 +27,7 @@
 namespace ctlab {
 
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef __bf16 bf16x4 __attribute__((ext_vector_type(4)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
 
@@ -196,7 +197,10 @@ k_conv32(const __bf16* __restrict__ in, const __bf16* __restrict__ w, const floa
     // channels of one voxel) of the channels-last output, reading the ReLU
     // mask of the input-gradient pass in the same chunks.  Writing the
     // fragments straight out took 64 two-byte stores per lane, ~20 % of the
-    // kernel at 512x512x64.
+    // kernel at 512x512x64.  (With the weights as the M operand each lane
+    // would hold 4 channels of one voxel, one 8-byte write per fragment; the
+    // kernel ran 2 % slower, and 3 % slower with the patch swizzled as in
+    // k_conv_first.)
     const long nvox = (long)XN * YN * ZN;
     __syncthreads();                                // every wavefront is done with sW / sA
     __bf16* sO = (__bf16*)sW + wv * 64 * CH;        // 64 voxels x 32 channels (4 KB) of this wavefront
@@ -249,7 +253,6 @@ k_conv32(const __bf16* __restrict__ in, const __bf16* __restrict__ w, const floa
 #pragma unroll
                 for (int i = 0; i < 4; ++i) o[i] = sO[(16 * b + 4 * kq + i) * CH + co];
                 if (zb + 3 < ZN && (ZN & 3) == 0) {     // v0 is a multiple of 4: 8-byte aligned vector store
-                    typedef __bf16 bf16x4 __attribute__((ext_vector_type(4)));    // (one run when rb)
                     *(bf16x4*)(out_cm + cm_index(co, v0, nvox, rb)) = (bf16x4){o[0], o[1], o[2], o[3]};
                 } else {
                     for (int i = 0; i < 4; ++i)
@@ -258,6 +261,15 @@ k_conv32(const __bf16* __restrict__ in, const __bf16* __restrict__ w, const floa
             }
         }
     }
+}
+
+// A wavefront's output patch in LDS: 64 voxels x 32 channels, 64 bytes per
+// voxel, with the 16-byte chunk q of voxel vb at chunk q ^ ((vb >> 2) & 3), so
+// that the 8-byte fragment writes (lanes: 16 voxels x 2 halves of a chunk)
+// fall on distinct banks while each voxel's 64 bytes stay contiguous.
+__device__ __forceinline__ int patch_off(int vb, int c)
+{
+    return vb * CH + ((((c >> 3) ^ (vb >> 2)) & 3) << 3) + (c & 7);
 }
 
 // ---------------------------------------------------------------------------
@@ -281,7 +293,7 @@ k_conv32(const __bf16* __restrict__ in, const __bf16* __restrict__ w, const floa
 // kCM: also write the channel-major copy out_cm (training); a separate instance,
 // since its address arithmetic, held across the row loop, costs 41 registers.
 template <bool kCM>
-__global__ void __launch_bounds__(256)
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4)))
 k_conv_first(const float* __restrict__ x, const float* __restrict__ w1, const float* __restrict__ b1,
              __bf16* __restrict__ out_cl, __bf16* __restrict__ out_cm, int XN, int YN, int ZN, int rb)
 {
@@ -336,7 +348,12 @@ k_conv_first(const float* __restrict__ x, const float* __restrict__ w1, const fl
             const int t = 8 * kq + e;
             B[nb][e] = (__bf16)(t < 27 ? w1[t * CH + nb * 16 + m] : 0.0f);
         }
-    const float bias0 = b1[m], bias1 = b1[16 + m];
+    // this lane's 4 output channels 16nb + 4kq + i (weights as the M operand: D[channel][voxel])
+    float bc[2][4];
+#pragma unroll
+    for (int nb = 0; nb < 2; ++nb)
+#pragma unroll
+        for (int i = 0; i < 4; ++i) bc[nb][i] = b1[16 * nb + 4 * kq + i];
     const long nvox = (long)XN * YN * ZN;
     __bf16* sO = (__bf16*)sO8 + wv * 64 * CH;
 
@@ -362,8 +379,8 @@ k_conv_first(const float* __restrict__ x, const float* __restrict__ w1, const fl
             off[e] = ((s + 1 + dy + 3) % 3) * kBrick + dx * kBZ + dz;
         }
 
-        // block b: column 2wv + (b >> 1), z 16(b & 1) + m.  Its fragments (lane: voxels
-        // 4kq + i, channel 16nb + m) go with bias and ReLU into this wavefront's
+        // block b: column 2wv + (b >> 1), z 16(b & 1) + m.  Its fragments (lane: channels
+        // 16nb + 4kq + i of voxel m) go with bias and ReLU into this wavefront's
         // [voxel][channel] patch, then 16-byte chunks out.  LDS executes one wavefront's
         // accesses in issue order, so the patch needs no barrier between rows.
 #pragma unroll
@@ -381,13 +398,13 @@ k_conv_first(const float* __restrict__ x, const float* __restrict__ w1, const fl
 #pragma unroll
             for (int nb = 0; nb < 2; ++nb) {
                 f32x4 acc = (f32x4){0.f, 0.f, 0.f, 0.f};
-                acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(hi, B[nb], acc, 0, 0, 0);
-                acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(mid, B[nb], acc, 0, 0, 0);
-                acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(lo, B[nb], acc, 0, 0, 0);
-                const float bco = nb ? bias1 : bias0;
+                acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(B[nb], hi, acc, 0, 0, 0);
+                acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(B[nb], mid, acc, 0, 0, 0);
+                acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(B[nb], lo, acc, 0, 0, 0);
+                bf16x4 o4;
 #pragma unroll
-                for (int i = 0; i < 4; ++i)
-                    sO[(16 * b + 4 * kq + i) * CH + nb * 16 + m] = (__bf16)fmaxf(acc[i] + bco, 0.0f);
+                for (int i = 0; i < 4; ++i) o4[i] = (__bf16)fmaxf(acc[i] + bc[nb][i], 0.0f);
+                *(bf16x4*)(sO + patch_off(16 * b + m, 16 * nb + 4 * kq)) = o4;
             }
         }
 #pragma unroll
@@ -398,7 +415,7 @@ k_conv_first(const float* __restrict__ x, const float* __restrict__ w1, const fl
             const int z = z0 + (vb & 31);
             if (xx >= XN || z >= ZN) continue;
             const long v = ((long)y * XN + xx) * ZN + z;
-            *(bf16x8*)(out_cl + v * CH + 8 * q) = *(const bf16x8*)(sO + vb * CH + 8 * q);
+            *(bf16x8*)(out_cl + v * CH + 8 * q) = *(const bf16x8*)(sO + patch_off(vb, 8 * q));
         }
         if (kCM) {
 #pragma unroll
@@ -412,9 +429,8 @@ k_conv_first(const float* __restrict__ x, const float* __restrict__ w1, const fl
                     const int co = nb * 16 + m;
                     __bf16 o[4];
 #pragma unroll
-                    for (int i = 0; i < 4; ++i) o[i] = sO[(16 * b + 4 * kq + i) * CH + co];
+                    for (int i = 0; i < 4; ++i) o[i] = sO[patch_off(16 * b + 4 * kq + i, co)];
                     if (zb + 3 < ZN && (ZN & 3) == 0) {
-                        typedef __bf16 bf16x4 __attribute__((ext_vector_type(4)));
                         *(bf16x4*)(out_cm + cm_index(co, v0, nvox, rb)) = (bf16x4){o[0], o[1], o[2], o[3]};
                     } else {
                         for (int i = 0; i < 4; ++i)
