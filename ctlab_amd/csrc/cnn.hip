@@ -13,7 +13,7 @@
 
 namespace ctlab {
 __global__ void k_conv32(const __bf16*, const __bf16*, const float*, const __bf16*, __bf16*, __bf16*, int, int,
-                         int, int, int);
+                         int, int, int, int);
 template <bool kCM>
 __global__ void k_conv_first(const float*, const float*, const float*, __bf16*, __bf16*, int, int, int, int);
 __global__ void k_conv_last(const __bf16*, const float*, const float*, const float*, float*, const float*, float*,
@@ -129,6 +129,7 @@ int forward(ctlab_cnn* N, const float* x, float* out, const float* target, hipSt
     const int XN = N->XN, YN = N->YN, ZN = N->ZN;
     const bool keep_cm = N->training;
     const int rb = N->nsweep > 0;       // run-blocked channel-major copies
+    const int nyb = conv32_rows(XN, YN, ZN);
     // layer 1
     if (keep_cm)
         hipLaunchKernelGGL(k_conv_first<true>, dim3(conv_first_grid(XN, YN, ZN)), dim3(256), 0, s, x, N->wr,
@@ -141,8 +142,8 @@ int forward(ctlab_cnn* N, const float* x, float* out, const float* target, hipSt
         const __bf16* in = N->training ? N->a_cl[l - 1] : N->a_cl[(l - 1) & 1];
         __bf16* ocl = N->training ? N->a_cl[l] : N->a_cl[l & 1];
         __bf16* ocm = N->training ? N->a_cm[l] : nullptr;
-        hipLaunchKernelGGL(k_conv32, dim3(conv32_grid(XN, YN, ZN)), dim3(256), 0, s, in, N->wb + (l - 1) * kW32,
-                           N->p + kOffW[l] + kLenW[l], (const __bf16*)nullptr, ocl, ocm, XN, YN, ZN, 1, rb);
+        hipLaunchKernelGGL(k_conv32, dim3(conv32_grid(XN, YN, ZN, nyb)), dim3(512), 0, s, in, N->wb + (l - 1) * kW32,
+                           N->p + kOffW[l] + kLenW[l], (const __bf16*)nullptr, ocl, ocm, XN, YN, ZN, 1, rb, nyb);
     }
     const __bf16* a4 = N->training ? N->a_cl[3] : N->a_cl[1];
     hipLaunchKernelGGL(k_conv_last, dim3(conv_last_grid(XN, YN, ZN)), dim3(256), 0, s, a4, N->wr + 27 * CH,
@@ -173,6 +174,7 @@ int backward(ctlab_cnn* N, const float* x, hipStream_t s)
     const int XN = N->XN, YN = N->YN, ZN = N->ZN;
     const long n = N->nvox;
     const int rb = N->nsweep > 0;       // run-blocked channel-major copies
+    const int nyb = conv32_rows(XN, YN, ZN);
     float* G = N->g;
     // layer 5
     hipLaunchKernelGGL(k_wgrad_last, dim3((unsigned)N->nvchunks), dim3(256), 0, s, N->g5, N->a_cl[3], N->part, XN, YN,
@@ -199,8 +201,9 @@ int backward(ctlab_cnn* N, const float* x, hipStream_t s)
             reduce_chunks(N, N->part, (int)N->nvchunks, CH, G + kOffW[l] + kLenW[l], s);
         }
         // input gradient through layer l+1, masked by the ReLU of layer l
-        hipLaunchKernelGGL(k_conv32, dim3(conv32_grid(XN, YN, ZN)), dim3(256), 0, s, N->g_cl[cur], N->wt + (l - 1) * kW32,
-                           (const float*)nullptr, N->a_cl[l - 1], N->g_cl[cur ^ 1], N->g_cm[cur ^ 1], XN, YN, ZN, 0, rb);
+        hipLaunchKernelGGL(k_conv32, dim3(conv32_grid(XN, YN, ZN, nyb)), dim3(512), 0, s, N->g_cl[cur],
+                           N->wt + (l - 1) * kW32, (const float*)nullptr, N->a_cl[l - 1], N->g_cl[cur ^ 1],
+                           N->g_cm[cur ^ 1], XN, YN, ZN, 0, rb, nyb);
         cur ^= 1;
     }
     // layer 1
@@ -229,8 +232,11 @@ ctlab_cnn* ctlab_cnn_create(int XN, int YN, int ZN, int training, int gpunum)
         set_error(CTLAB_ERR_ARG, "training needs ZN %% 8 == 0 (got %d)", ZN);
         return nullptr;
     }
-    // one-dimensional launch grids of the tiled convolution kernels
-    if (conv32_tiles(XN, YN, ZN) > 0x7FFFFFF0L || conv_last_tiles(XN, YN, ZN) > 0x7FFFFFF0L) {
+    // one-dimensional launch grids of the tiled convolution kernels; the 32 -> 32
+    // layers address one y row of channels-last activations (64 bytes per voxel)
+    // with 32-bit offsets
+    if (conv32_tiles(XN, YN, ZN, 2) > 0x7FFFFFF0L || conv_last_tiles(XN, YN, ZN) > 0x7FFFFFF0L ||
+        conv_first_tiles(XN, YN, ZN) > 0x7FFFFFF0L || (long)XN * ZN * 64 > 0x7FFFFFFFL) {
         set_error(CTLAB_ERR_ARG, "volume %d x %d x %d exceeds the convolution kernels' grid", XN, YN, ZN);
         return nullptr;
     }

@@ -55,27 +55,28 @@ __device__ __forceinline__ long cm_index(int c, long v, long nvox, int rb)
 }
 
 // ---------------------------------------------------------------------------
-// 32 -> 32 convolution, MFMA.  Block = 4 wavefronts, output tile = 8 x-columns
-// x 32 z of one y row (256 voxels; each wavefront two columns, 4 blocks of 16
-// voxels along z).
+// 32 -> 32 convolution, MFMA (implicit GEMM: voxels x 27 taps x 32 channels).
 //   out = act(conv(in; w) + bias) ; act: relu (relu=1) or identity
 //   mask (optional, channels-last): out *= (mask > 0)   [input-gradient pass]
 // Writes channels-last and (optionally) channel-major copies.
-// Per dy plane, the tile's input brick (10 columns x 34 z of row y+dy, all 32
-// channels, zero outside the volume) and the plane's 9 taps of weights are
-// staged in LDS (22.5 + 18.4 KB: 4 blocks per CU); the 9 taps then read their
-// MFMA operands from LDS, so each input voxel is fetched from L2 about 1.3
-// times per plane instead of 9.  The next plane's brick is loaded into
-// registers while the current plane's MFMAs run (a plane's 72 MFMAs per
-// wavefront are shorter than one round of global loads; fetching after the
-// barrier left the MFMAs idle: 1.52 -> 1.16 ms per layer at 512x512x64).
-// Prefetching the weights into registers as well spills at 128 registers
-// (2x slower).  Registers are held to 128 (4 wavefronts per SIMD, what the LDS
-// allows): 136 registers and 3 wavefronts ran 14 % slower.  What bounds it
-// (37-39 % MFMA busy at 512x512x64, no LDS bank conflicts): the MFMAs of a
-// tap wait on their LDS operands (ds_read, s_waitcnt lgkmcnt(0), 1-2 MFMAs),
-// the 128-register budget leaving no room to read a tap ahead.  Tiles are
-// dealt to the 8 XCDs in contiguous ranges of the band order below.
+// Block = 8 wavefronts, one per CU (LDS); output tile = 8 x-columns x 32 z,
+// for nyb consecutive y rows, taken two rows at a time: wavefronts 0-3 compute
+// row y, 4-7 row y + 1 (each wavefront two x-columns, 4 blocks of 16 voxels
+// along z, 8 accumulator chains).  All 27 taps' weights stay in LDS for the
+// block's life.  The input planes (bricks: 10 columns x 34 z of one y row, all
+// 32 channels, zero outside the volume) pass through a ring of three LDS
+// slots.  A pair of rows takes three steps; at step k the row-y group
+// multiplies plane y - 1 + k and the row-(y+1) group plane y + k, both with
+// weight plane k, so the two groups share every brick and a pair needs two new
+// bricks: one brick per output row, where one row per block with per-plane
+// weight staging took three bricks and three weight planes.  Bricks arrive by
+// LDS DMA (buffer_load_dwordx4 ... lds), issued one step before they are read,
+// with one barrier per step.  The DMA writes lane i of an instruction at
+// M0 + 16 i, so an instruction fills 64 consecutive rows of one chunk plane;
+// loads outside the row's buffer range write zeros, which gives the halo.
+// The ring slots are separate LDS arrays read without a select, so that the
+// compiler can tell a slot being filled from the ones being read and issues
+// no vmcnt wait between the DMA and the MFMA operand reads.
 // ---------------------------------------------------------------------------
 constexpr int kBX = kTX + 2, kBZ = kTZ + 2;         // input brick of one plane
 constexpr int kBrick = kBX * kBZ;                   // 340 voxels
@@ -87,179 +88,189 @@ constexpr int kBrick = kBX * kBZ;                   // 340 voxels
 // (ds_read_b128, 64 lanes as here): a read takes 4 LDS cycles when the plane
 // stride is a multiple of 256 bytes, and 8 (2-way bank conflicts) with 64-byte
 // rows (the layout [row][chunk]) or with a plane stride of 5440, 4160 or 4224
-// bytes.  Planes are filled in the same (row, chunk) lane pattern they are
-// read in: staging index i holds row 16 * (i / 64) + i % 16, chunk (i / 16) % 4.
-constexpr int kBrickP = 352;                        // brick plane stride: 5,632 B
-constexpr int kXBand = 16;                          // x-tiles per band of the tile order
+// bytes.  Register-staged planes are filled in the same (row, chunk) lane
+// pattern they are read in: staging index i holds row 16 * (i / 64) + i % 16,
+// chunk (i / 16) % 4.
+constexpr int kBrickP = 352;                        // brick plane stride: 5,632 B (register-staged)
+constexpr int kDmaP = 384;                          // brick plane stride for DMA: 6 x 64 rows (6,144 B)
+constexpr int kSlot = 4 * kDmaP;                    // one DMA brick: 24,576 B
+constexpr int kWPA = 27 * CH;                       // weight plane stride, all taps: 864 rows (13,824 B)
 
-// Tile order of k_conv32: bands of kXBand x-tiles (128 columns); within a band
-// y, then x, then z.  An XCD's concurrent blocks then cover a few y rows of
-// one band, whose input rows (y-1 .. y+1) stay in its L2, instead of whole
-// rows (512 x 512 x 64: three 2 MB input rows per output row); 4 % faster.
-// Bands past the last full one are narrower (ntx % kXBand tiles).
-__device__ __forceinline__ void conv32_band_tile(long r, int ntx, int YN, int& xt, int& y)
-{
-    const int nfull = ntx / kXBand;
-    const long fullt = (long)nfull * kXBand * YN;
-    if (r < fullt) {
-        const long bb = r / ((long)kXBand * YN), rr = r - bb * kXBand * YN;
-        y = (int)(rr / kXBand);
-        xt = (int)(bb * kXBand + rr % kXBand);
-    } else {
-        const int wl = ntx - nfull * kXBand;
-        const long rr = r - fullt;
-        y = (int)(rr / wl);
-        xt = nfull * kXBand + (int)(rr % wl);
-    }
-}
-constexpr int kWP = 9 * CH;                         // weight plane stride: 4,608 B
-
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4)))
+__global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(2)))
 k_conv32(const __bf16* __restrict__ in, const __bf16* __restrict__ w, const float* __restrict__ bias,
          const __bf16* __restrict__ mask, __bf16* __restrict__ out_cl, __bf16* __restrict__ out_cm,
-         int XN, int YN, int ZN, int relu, int rb)
+         int XN, int YN, int ZN, int relu, int rb, int nyb)
 {
-    __shared__ bf16x8 sW[4 * kWP];                  // 18,432 B: the 9 taps of one dy plane
-    __shared__ bf16x8 sA[4 * kBrickP];              // 22,528 B: the input brick of one plane
-    const long ntiles = conv32_tiles(XN, YN, ZN);
-    // contiguous tile ranges per XCD (blocks are dealt to XCDs round-robin); the
-    // grid is 8 * ceil(ntiles / 8), so (xcd, j) covers [0, 8 * per) once
+    __shared__ bf16x8 sW[4 * kWPA];                 // 55,296 B: all 27 taps
+    __shared__ bf16x8 sB0[kSlot], sB1[kSlot], sB2[kSlot];   // 73,728 B: ring of three bricks
+    __shared__ bf16x8 sO8[8 * 64 * CH / 8];         // 32,768 B: each wavefront's output patch
+    const long ntiles = conv32_tiles(XN, YN, ZN, nyb);
+    // contiguous tile ranges per XCD (blocks are dealt to XCDs round-robin); the grid
+    // is 8 * ceil(ntiles / 8), so (xcd, j) covers [0, 8 * per) once.  Tile order: z,
+    // then x, then y range, so an XCD's concurrent blocks share their halo columns in L2.
     const long per = (long)gridDim.x >> 3;
     const long tile = (long)(blockIdx.x & 7) * per + (blockIdx.x >> 3);
     if (tile >= ntiles) return;                     // whole block: before any barrier
     const int ntz = (ZN + kTZ - 1) / kTZ, ntx = (XN + kTX - 1) / kTX;
     const int zt = (int)(tile % ntz);
-    int xt, y;
-    conv32_band_tile(tile / ntz, ntx, YN, xt, y);
-    const int x0 = xt * kTX, z0 = zt * kTZ;
+    const long r = tile / ntz;
+    const int x0 = (int)(r % ntx) * kTX, z0 = zt * kTZ;
+    const int y0 = (int)(r / ntx) * nyb, ny = min(nyb, YN - y0);
 
     const int lane = threadIdx.x & 63;
-    const int wv = threadIdx.x >> 6;
+    const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int grp = wv >> 2, w4 = wv & 3;           // row y + grp; x-columns 2 w4, 2 w4 + 1
     const int m = lane & 15, kq = lane >> 4;
-    // block b of this wavefront: column 2*wv + (b >> 1), z 16*(b & 1) + m; its
+    // block b of this wavefront: column 2*w4 + (b >> 1), z 16*(b & 1) + m; its
     // centre voxel in the brick is (column + 1, z + 1)
     int pb[4];
 #pragma unroll
-    for (int b = 0; b < 4; ++b) pb[b] = (2 * wv + (b >> 1) + 1) * kBZ + 16 * (b & 1) + m + 1;
+    for (int b = 0; b < 4; ++b) pb[b] = (2 * w4 + (b >> 1) + 1) * kBZ + 16 * (b & 1) + m + 1;
 
-    f32x4 acc[4][2];
+    // A brick is 24 DMA instructions (chunk plane q, rows 64 j .. 64 j + 63); wavefront
+    // wv issues numbers wv, wv + 8, wv + 16.  Lane offsets are taken within one y row of
+    // the input, whose buffer range is the row; rows past the brick, outside the volume
+    // in x or z, or of a y row outside it read past the range and write zeros.
+    const unsigned rowbytes = (unsigned)XN * (unsigned)ZN * 64u;
+    unsigned loff[3];
 #pragma unroll
-    for (int b = 0; b < 4; ++b) acc[b][0] = acc[b][1] = (f32x4){0.f, 0.f, 0.f, 0.f};
-
-    const bf16x8 zero8 = (bf16x8){0, 0, 0, 0, 0, 0, 0, 0};
-    constexpr int kNA = (kBrickP * CH / 8 + 255) / 256;
-    bf16x8 pa[kNA];
-    // staging index i -> (row, chunk), the lane pattern of the operand reads
-    auto srow = [](int i) { return ((i >> 6) << 4) | (i & 15); };
-    auto schunk = [](int i) { return (i >> 4) & 3; };
-    auto fetch = [&](int plane) {
-        const int yy = y + plane - 1;
+    for (int e = 0; e < 3; ++e) {
+        const int I = wv + 8 * e, q = I / 6, p = 64 * (I % 6) + lane;
+        const int gx = x0 - 1 + p / kBZ, gz = z0 - 1 + p % kBZ;
+        const bool ok = p < kBrick && (unsigned)gx < (unsigned)XN && (unsigned)gz < (unsigned)ZN;
+        loff[e] = ok ? (unsigned)(((gx * ZN + gz) * 4 + q) * 16) : rowbytes;
+    }
+    auto load_brick = [&](int n, bf16x8* slot) {    // plane n: input row y0 - 1 + n
+        const int yy = y0 - 1 + n;
         const bool row_in = (unsigned)yy < (unsigned)YN;
+        const __bf16* base = in + (row_in ? (long)yy * XN * ZN * CH : 0L);
+        const __amdgpu_buffer_rsrc_t rs =
+            __builtin_amdgcn_make_buffer_rsrc((void*)base, (short)0, row_in ? rowbytes : 0u, 0x00020000);
 #pragma unroll
-        for (int j = 0; j < kNA; ++j) {
-            const int i = threadIdx.x + 256 * j;
-            const int p = srow(i), q = schunk(i);
-            const int gx = x0 - 1 + p / kBZ, gz = z0 - 1 + p % kBZ;
-            pa[j] = zero8;
-            if (p < kBrick && row_in && (unsigned)gx < (unsigned)XN && (unsigned)gz < (unsigned)ZN)
-                pa[j] = ((const bf16x8*)in)[(((long)yy * XN + gx) * ZN + gz) * (CH / 8) + q];
+        for (int e = 0; e < 3; ++e) {
+            const int I = wv + 8 * e;
+            __builtin_amdgcn_raw_ptr_buffer_load_lds(
+                rs, (__attribute__((address_space(3))) void*)(slot + (I / 6) * kDmaP + 64 * (I % 6)), 16, loff[e], 0, 0, 0);
         }
     };
-    fetch(0);
-    for (int plane = 0; plane < 3; ++plane) {
-        __syncthreads();   // the previous plane's operands are no longer read
-        for (int i = threadIdx.x; i < 9 * CH * CH / 8; i += 256)
-            sW[schunk(i) * kWP + srow(i)] = ((const bf16x8*)w)[plane * 9 * CH * CH / 8 + srow(i) * 4 + schunk(i)];
-#pragma unroll
-        for (int j = 0; j < kNA; ++j) {
-            const int i = threadIdx.x + 256 * j;
-            if (srow(i) < kBrick) sA[schunk(i) * kBrickP + srow(i)] = pa[j];
-        }
-        __syncthreads();
-        if (plane < 2) fetch(plane + 1);   // in flight during this plane's MFMAs
+    load_brick(0, sB0);
+    load_brick(1, sB1);
+    auto srow = [](int i) { return ((i >> 6) << 4) | (i & 15); };
+    auto schunk = [](int i) { return (i >> 4) & 3; };
+    for (int i = threadIdx.x; i < 27 * CH * CH / 8; i += 512)
+        sW[schunk(i) * kWPA + srow(i)] = ((const bf16x8*)w)[srow(i) * 4 + schunk(i)];
+
+    const long nvox = (long)XN * YN * ZN;
+    __bf16* sO = (__bf16*)sO8 + wv * 64 * CH;       // 64 voxels x 32 channels (4 KB) of this wavefront
+    const float bias0 = bias ? bias[m] : 0.0f, bias1 = bias ? bias[16 + m] : 0.0f;
+    f32x4 acc[4][2];
+    auto taps = [&](const bf16x8* sA, int k) {      // the 9 taps of weight plane k (dy = k - 1) on brick sA
 #pragma unroll
         for (int tt = 0; tt < 9; ++tt) {
             const int shift = (tt / 3 - 1) * kBZ + (tt % 3 - 1);      // (dx, dz) in the brick
-            const bf16x8 b0 = sW[kq * kWP + tt * CH + m];
-            const bf16x8 b1 = sW[kq * kWP + tt * CH + 16 + m];
+            const int wr = (9 * k + tt) * CH;
+            const bf16x8 b0 = sW[kq * kWPA + wr + m];
+            const bf16x8 b1 = sW[kq * kWPA + wr + 16 + m];
 #pragma unroll
             for (int b = 0; b < 4; ++b) {
-                const bf16x8 a = sA[kq * kBrickP + pb[b] + shift];
+                const bf16x8 a = sA[kq * kDmaP + pb[b] + shift];
                 acc[b][0] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b0, acc[b][0], 0, 0, 0);
                 acc[b][1] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b1, acc[b][1], 0, 0, 0);
             }
         }
-    }
-
-    // Epilogue through LDS.  The D fragments (lane: 4 voxels of one output
-    // channel) go, with bias and ReLU, into this wavefront's [voxel][channel]
-    // patch of the free weight tile; each lane then writes 16-byte chunks (8
-    // channels of one voxel) of the channels-last output, reading the ReLU
-    // mask of the input-gradient pass in the same chunks.  Writing the
-    // fragments straight out took 64 two-byte stores per lane, ~20 % of the
-    // kernel at 512x512x64.  (With the weights as the M operand each lane
-    // would hold 4 channels of one voxel, one 8-byte write per fragment; the
-    // kernel ran 2 % slower, and 3 % slower with the patch swizzled as in
-    // k_conv_first.)
-    const long nvox = (long)XN * YN * ZN;
-    __syncthreads();                                // every wavefront is done with sW / sA
-    __bf16* sO = (__bf16*)sW + wv * 64 * CH;        // 64 voxels x 32 channels (4 KB) of this wavefront
-#pragma unroll
-    for (int b = 0; b < 4; ++b) {
-#pragma unroll
-        for (int nb = 0; nb < 2; ++nb) {
-            const int co = nb * 16 + m;
-            const float bco = bias ? bias[co] : 0.0f;
-#pragma unroll
-            for (int i = 0; i < 4; ++i) {
-                float rr = acc[b][nb][i] + bco;
-                if (relu) rr = fmaxf(rr, 0.0f);
-                sO[(16 * b + 4 * kq + i) * CH + co] = (__bf16)rr;
-            }
-        }
-    }
-    // LDS executes one wavefront's accesses in issue order: the chunks below
-    // see the values this wavefront's lanes wrote above
-#pragma unroll
-    for (int k = 0; k < 4; ++k) {
-        const int c = lane + 64 * k;                // chunk: voxel c / 4, channels 8 * (c % 4) ..
-        const int vb = c >> 2, q = c & 3;
-        const int x = x0 + 2 * wv + (vb >> 5);
-        const int z = z0 + (vb & 31);
-        if (x >= XN || z >= ZN) continue;
-        const long v = ((long)y * XN + x) * ZN + z;
-        bf16x8 o = *(const bf16x8*)(sO + vb * CH + 8 * q);
-        if (mask) {
-            const bf16x8 mk = *(const bf16x8*)(mask + v * CH + 8 * q);
-#pragma unroll
-            for (int e = 0; e < 8; ++e)
-                if (!(bf(mk[e]) > 0.0f)) o[e] = (__bf16)0.0f;
-            if (out_cm) *(bf16x8*)(sO + vb * CH + 8 * q) = o;   // the masked values for the copy below
-        }
-        *(bf16x8*)(out_cl + v * CH + 8 * q) = o;
-    }
-    if (out_cm) {
-        // channel-major (or run-blocked) copy: lane = (channel m of block nb, 4 voxels)
+    };
+    auto step_sync = [&]() {
+        __builtin_amdgcn_s_waitcnt(0x0f70);         // vmcnt(0): this wavefront's DMA has landed
+        __syncthreads();                            // everyone's; the slot to be refilled is free
+    };
+    // Epilogue of row y.  The D fragments (lane: 4 voxels of one output channel) go,
+    // with bias and ReLU, into this wavefront's [voxel][channel] patch; each lane then
+    // writes 16-byte chunks (8 channels of one voxel) of the channels-last output,
+    // reading the ReLU mask of the input-gradient pass in the same chunks.  The patch
+    // is the wavefront's own and LDS executes one wavefront's accesses in issue order,
+    // so it needs no barrier.  Writing the fragments straight out took 64 two-byte
+    // stores per lane, ~20 % of the kernel at 512x512x64.
+    auto epilogue = [&](int y) {
 #pragma unroll
         for (int b = 0; b < 4; ++b) {
-            const int x = x0 + 2 * wv + (b >> 1);
-            const int zb = z0 + 16 * (b & 1) + 4 * kq;
-            if (x >= XN) continue;
-            const long v0 = ((long)y * XN + x) * ZN + zb;
 #pragma unroll
             for (int nb = 0; nb < 2; ++nb) {
                 const int co = nb * 16 + m;
-                __bf16 o[4];
+                const float bco = nb ? bias1 : bias0;
 #pragma unroll
-                for (int i = 0; i < 4; ++i) o[i] = sO[(16 * b + 4 * kq + i) * CH + co];
-                if (zb + 3 < ZN && (ZN & 3) == 0) {     // v0 is a multiple of 4: 8-byte aligned vector store
-                    *(bf16x4*)(out_cm + cm_index(co, v0, nvox, rb)) = (bf16x4){o[0], o[1], o[2], o[3]};
-                } else {
-                    for (int i = 0; i < 4; ++i)
-                        if (zb + i < ZN) out_cm[cm_index(co, v0 + i, nvox, rb)] = o[i];
+                for (int i = 0; i < 4; ++i) {
+                    float rr = acc[b][nb][i] + bco;
+                    if (relu) rr = fmaxf(rr, 0.0f);
+                    sO[(16 * b + 4 * kq + i) * CH + co] = (__bf16)rr;
                 }
             }
         }
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const int c = lane + 64 * k;            // chunk: voxel c / 4, channels 8 * (c % 4) ..
+            const int vb = c >> 2, q = c & 3;
+            const int x = x0 + 2 * w4 + (vb >> 5);
+            const int z = z0 + (vb & 31);
+            if (x >= XN || z >= ZN) continue;
+            const long v = ((long)y * XN + x) * ZN + z;
+            bf16x8 o = *(const bf16x8*)(sO + vb * CH + 8 * q);
+            if (mask) {
+                const bf16x8 mk = *(const bf16x8*)(mask + v * CH + 8 * q);
+#pragma unroll
+                for (int e = 0; e < 8; ++e)
+                    if (!(bf(mk[e]) > 0.0f)) o[e] = (__bf16)0.0f;
+                if (out_cm) *(bf16x8*)(sO + vb * CH + 8 * q) = o;   // the masked values for the copy below
+            }
+            *(bf16x8*)(out_cl + v * CH + 8 * q) = o;
+        }
+        if (out_cm) {
+            // channel-major (or run-blocked) copy: lane = (channel m of block nb, 4 voxels)
+#pragma unroll
+            for (int b = 0; b < 4; ++b) {
+                const int x = x0 + 2 * w4 + (b >> 1);
+                const int zb = z0 + 16 * (b & 1) + 4 * kq;
+                if (x >= XN) continue;
+                const long v0 = ((long)y * XN + x) * ZN + zb;
+#pragma unroll
+                for (int nb = 0; nb < 2; ++nb) {
+                    const int co = nb * 16 + m;
+                    __bf16 o[4];
+#pragma unroll
+                    for (int i = 0; i < 4; ++i) o[i] = sO[(16 * b + 4 * kq + i) * CH + co];
+                    if (zb + 3 < ZN && (ZN & 3) == 0) {     // v0 is a multiple of 4: 8-byte aligned vector store
+                        *(bf16x4*)(out_cm + cm_index(co, v0, nvox, rb)) = (bf16x4){o[0], o[1], o[2], o[3]};
+                    } else {
+                        for (int i = 0; i < 4; ++i)
+                            if (zb + i < ZN) out_cm[cm_index(co, v0 + i, nvox, rb)] = o[i];
+                    }
+                }
+            }
+        }
+    };
+    // Pair t (rows y0 + 2t, y0 + 2t + 1): planes 2t in P and 2t + 1 in Q; plane 2t + 2 is
+    // loaded into R at step 0 (R last held plane 2t - 1, read at the previous pair's
+    // step 1) and plane 2t + 3 into P at step 1 (P's plane was last read at step 0).
+    // The loads past the last plane read rows beyond the tile, harmlessly.  Each group
+    // reads its slot through a wave-uniform branch rather than a selected pointer.
+    auto pair = [&](int t, bf16x8* P, bf16x8* Q, bf16x8* R) {
+#pragma unroll
+        for (int b = 0; b < 4; ++b) acc[b][0] = acc[b][1] = (f32x4){0.f, 0.f, 0.f, 0.f};
+        step_sync();
+        load_brick(2 * t + 2, R);
+        if (grp) taps(Q, 0); else taps(P, 0);
+        step_sync();
+        load_brick(2 * t + 3, P);
+        if (grp) taps(R, 1); else taps(Q, 1);
+        step_sync();
+        if (grp) taps(P, 2); else taps(R, 2);
+        const int y = y0 + 2 * t + grp;
+        if (y < y0 + ny) epilogue(y);
+    };
+    const int npair = (ny + 1) >> 1;
+    for (int t = 0; t < npair; t += 3) {            // slots rotate with period three pairs
+        pair(t, sB0, sB1, sB2);
+        if (t + 1 < npair) pair(t + 1, sB2, sB0, sB1);
+        if (t + 2 < npair) pair(t + 2, sB1, sB2, sB0);
     }
 }
 
