@@ -333,9 +333,11 @@ __device__ __forceinline__ T column_delta(const DevGeom& g, T* G, T Pu, double e
 }
 
 // Column record in walk order (increasing row coordinate a): slopes of the
-// column's lower and upper boundary rays, 1/|slope step|, and esdz.  Each view
-// holds DNU records and a sentinel (upper slope NaN, 1/|slope step| and esdz
-// 0), and one spare record follows the last view.
+// column's lower and upper boundary rays times sign(s0) (so that, with
+// |rs| = |slab distance| / dx, a = rbase + |rs| * slope grows with the slope),
+// 1/|slope step|, and esdz.  Each view holds DNU records and a sentinel (upper
+// slope NaN, 1/|slope step| and esdz 0), and one spare record follows the last
+// view.
 struct ColRec { double klo, khi, invdk, esdz; };
 
 // buffer-resource word 3 for raw (untyped, unswizzled) buffer access on gfx9
@@ -386,7 +388,6 @@ __device__ __forceinline__ void back_walk_accum(const DevGeom& g, const CTLAB_GM
     const int nm = min(NM, NR - r0);
     const int DNU = g.DNU, DNV = g.DNV;
     const int vl = min(lane, DNV - 1);
-    const double inv_dx = 1.0 / g.dx;
     const long srow = (long)j * SR + (r0 >> 3);
     const T sgn = g.dzh < 0.0 ? T(-1) : T(1);
     const double zoff = g.zhat0 / g.dzh;
@@ -426,11 +427,14 @@ __device__ __forceinline__ void back_walk_accum(const DevGeom& g, const CTLAB_GM
         if (iv + 3 < nviews) p3 = views[iv + 3];
         if (st == 0xFFFF) continue;
         const ViewParams vp = g.vp[p];
-        const double sj = vp.s0 + j * g.dx;
-        const double rs = sj * inv_dx;
         const double inv_sj = g.invs[(long)p * g.sjs + j];
         const double inv_rs = fabs(g.dx * inv_sj);
         const double lzb = ((double)lane - vp.beta0) * inv_sj;
+        // Row boundary m in slope space: km = (m - rbase) / |rs|.  It lies at or
+        // above column c's upper edge when km >= khi(c), and its fraction of the
+        // column is (km - klo) / (khi - klo); both read the record's values as
+        // they are, with no per-column FP64 set-up.
+        double km = ((double)r0 - vp.rbase) * inv_rs;
         // walk-order column c = st: its record (its v-prefix entries and the
         // next column's came through the pipeline); each further column's are
         // loaded one column ahead.  The view's v-prefix rows are read through
@@ -442,10 +446,8 @@ __device__ __forceinline__ void back_walk_accum(const DevGeom& g, const CTLAB_GM
         unsigned voff = (unsigned)((vp.asc ? st : DNU - 1 - st) * DNV + vl) * (unsigned)sizeof(T);
         const unsigned vstep = (unsigned)((vp.asc ? DNV : -DNV) * (int)sizeof(T));
         ColRec r = rq[0];
-        double alo = vp.rbase + rs * r.klo;
-        if (alo >= (double)(r0 + nm)) continue;
-        double ahi = vp.rbase + rs * r.khi;
-        double iw = inv_rs * r.invdk;
+        if (r.klo >= fma((double)nm, inv_rs, km)) continue;
+        double klo = r.klo, khi = r.khi, iw = r.invdk;
         rq += 1;
         voff += vstep;
         ColRec rn = rq[0];
@@ -453,26 +455,26 @@ __device__ __forceinline__ void back_walk_accum(const DevGeom& g, const CTLAB_GM
         T Tsum = T(0), Wprev = T(0);
         // not unrolled (each copy would carry its own column walk)
         for (int i = 0; i <= nm; ++i) {
-            const double mp = (double)(r0 + i);
-            // advance while the column ends at or below row boundary r0 + i
-            // (for an integer boundary the same as r0 + i >= ceil(ahi)); the
-            // sentinel's NaN upper edge stops the walk after the last column,
-            // whose term has then moved into Tsum while the sentinel adds 0
-            while (mp >= ahi) {
+            // advance while the column ends at or below row boundary r0 + i;
+            // the sentinel's NaN upper edge stops the walk after the last
+            // column, whose term has then moved into Tsum while the sentinel
+            // adds 0 (its 1/|slope step| is 0)
+            while (km >= khi) {
                 Tsum += D;
-                alo = ahi;
-                ahi = vp.rbase + rs * rn.khi;
-                iw = inv_rs * rn.invdk;
+                klo = khi;
+                khi = rn.khi;
+                iw = rn.invdk;
                 D = column_delta<ZM>(g, G, Pr1, rn.esdz, lzb, zoff, inv_sj, vp.beta0, vp.top_total, lane);
                 rq += 1;
                 voff += vstep;
                 rn = rq[0];
                 Pr1 = buf_load<T>(prs, voff);
             }
-            const double fr = fmin(fmax((mp - alo) * iw, 0.0), 1.0);
+            const double fr = fmin(fmax((km - klo) * iw, 0.0), 1.0);
             const T W = Tsum + (T)fr * D;
             A[(i - 1) * 64 + lane] += (AccT)(W - Wprev);
             Wprev = W;
+            km += inv_rs;
         }
     }
     // the z-differences above are taken along the detector row axis; sgn

@@ -92,8 +92,9 @@ back_start_kernel(DevGeom g, const ColRec* __restrict__ rec, unsigned short* __r
     const int DNU = g.DNU;
     const double r0 = 8.0 * r8;
     if (j >= vp.NS || r0 >= vp.NR) { start[idx] = 0xFFFF; return; }
-    const double rs = (vp.s0 + j * g.dx) / g.dx;
+    const double rs = fabs((vp.s0 + j * g.dx) / g.dx);
     const ColRec* R = rec + (long)p * (DNU + 1);
+    // the records' slopes carry the sign of the source-to-slab distance
     auto A = [&](int q) { return vp.rbase + rs * (q < DNU ? R[q].klo : R[DNU - 1].khi); };
     if (A(DNU) <= r0) { start[idx] = 0xFFFF; return; }
     int lo = 1, hi = DNU;

@@ -256,12 +256,17 @@ static int build_tables(ctlab_plan* P)
             V.top_total = lo >= (double)DNV;
         }
         (yslab ? P->views_y : P->views_x).push_back(p);
-        // walk-order (increasing row coordinate) column records for the backprojector
+        // walk-order (increasing row coordinate) column records for the backprojector.
+        // Slopes are stored times the sign of the source-to-slab distance, which is
+        // the same for every slab of the view: a boundary with stored slope k' lies
+        // at row coordinate a = rbase + |rs| * k', so k' grows in walk order and
+        // the walk compares row boundaries with column edges in slope space.
+        const double sgn_s = V.s0 > 0 ? 1.0 : -1.0;
         for (int c = 0; c < DNU; ++c) {
             const int u = V.asc ? c : DNU - 1 - c;
             double* r = &P->rec[((size_t)p * (DNU + 1) + c) * 4];
-            r[0] = V.asc ? kp[u] : kp[u + 1];
-            r[1] = V.asc ? kp[u + 1] : kp[u];
+            r[0] = sgn_s * (V.asc ? kp[u] : kp[u + 1]);
+            r[1] = sgn_s * (V.asc ? kp[u + 1] : kp[u]);
             r[2] = P->invdk[(size_t)p * DNU + u];
             r[3] = P->esdz[(size_t)p * DNU + u];
         }

@@ -68,7 +68,7 @@ struct ctlab_plan {
     double *d_zhat = nullptr, *d_ez = nullptr, *d_rowfac = nullptr;
     unsigned char* d_mask = nullptr;
     int* d_views = nullptr;
-    std::vector<double> rec;            // (PN*(DNU+1)+1)*4: walk-order column records (klo, khi, 1/|dk|, esdz), sentinel per view
+    std::vector<double> rec;            // (PN*(DNU+1)+1)*4: walk-order column records (sign(s0)*klo, sign(s0)*khi, 1/|dk|, esdz), sentinel per view
     double* d_rec = nullptr;
     unsigned short* d_start = nullptr;  // PN*SJ*SR first walk-order column per (view, slab, 8 rows)
     int SJ = 0, SR = 0;
