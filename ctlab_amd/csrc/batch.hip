@@ -139,8 +139,12 @@ __global__ void __launch_bounds__(256) bk_proj(const BatchProb* __restrict__ pb,
     const int lane = threadIdx.x & 63;
     drain<1>(q, [&](int k, int it) {
         const BatchProb b = load_prob(pb + k);
-        proj_walk_wave<T, NU>(b.g, (const CTLAB_GMEM T*)b.cz, b.prj, it % b.g.PN, it / b.g.PN, lane, sG[wv],
-                              sFT[wv]);
+        if (b.g.cone)
+            proj_walk_wave<T, NU, true>(b.g, (const CTLAB_GMEM T*)b.cz, b.prj, it % b.g.PN, it / b.g.PN, lane,
+                                        sG[wv], sFT[wv]);
+        else
+            proj_walk_wave<T, NU, false>(b.g, (const CTLAB_GMEM T*)b.cz, b.prj, it % b.g.PN, it / b.g.PN, lane,
+                                         sG[wv], sFT[wv]);
     });
 }
 

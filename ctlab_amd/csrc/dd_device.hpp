@@ -133,7 +133,7 @@ constexpr int kGStride = 68;
 // (Reading no row past the walk's last one saves L2 traffic but measured 18 %
 // slower: the branch around each read breaks the load pipelining.)  Only the
 // slabs the tile can reach are walked, an interval of slabs found once per tile.
-template <typename T, int NU>
+template <typename T, int NU, bool CONE>
 __device__ __forceinline__ void proj_walk_wave(const DevGeom& g, const CTLAB_GMEM T* __restrict__ cz,
                                                CTLAB_GMEM float* __restrict__ prj,
                                                int p, int tile, int lane, T* G, double* FT)
@@ -196,23 +196,27 @@ __device__ __forceinline__ void proj_walk_wave(const DevGeom& g, const CTLAB_GME
         bound(readlane_d(kap_l, bl), 0.0, true);
         bound(readlane_d(kap_l, b0), (double)NR, false);
     }
-    for (int j = j_lo; j <= j_hi; ++j) {
-        const double sj = vp.s0 + j * g.dx;
-        const double rs = sj * inv_dx;
-        // row coordinate of every tile boundary at once (lane = boundary),
-        // clamped to the slab; its row index and fraction within the row
-        const double a_l = fmin(fmax(vp.rbase + rs * kap_l, 0.0), (double)NR);
+    if constexpr (!CONE) FT[2 * kt + 1] = 1.0;   // multi-slice fan beam: no z magnification
+    // slab j's centre relative to the source, and every boundary's row
+    // coordinate at slab j (lane = boundary), both advanced one slab at a time
+    double sj = vp.s0 + j_lo * g.dx;
+    double a_raw = vp.rbase + (vp.s0 * inv_dx + j_lo) * kap_l;
+    for (int j = j_lo; j <= j_hi; ++j, sj += g.dx, a_raw += kap_l) {
+        // row coordinate clamped to the slab; its row index and fraction within the row
+        const double a_l = fmin(fmax(a_raw, 0.0), (double)NR);
         const int m_l = min((int)a_l, NR - 1);
         const double f_l = a_l - (double)m_l;
         FT[2 * kf] = f_l;
-        FT[2 * kt + 1] = g.cone ? sj * tau_l : 1.0;   // z magnification of column `lane`
-        // columns (lane = tile column u, boundaries u and u+1) with a positive row extent
+        if constexpr (CONE) FT[2 * kt + 1] = sj * tau_l;   // z magnification of column `lane`
+        // columns (lane = tile column u, boundaries u and u+1) with a positive
+        // row extent: the unclamped coordinates differ, so the clamped ones
+        // differ where the column overlaps the slab
         const double a_n = shift_down1(a_l);
-        const unsigned act = (unsigned)__ballot(lane < nu && (vp.asc ? a_n > a_l : a_l > a_n));
+        const unsigned act = (unsigned)__ballot(lane < nu && a_n != a_l);
         // the same bits in walk order: bit k = the column crossed at step k
         const unsigned actk = vp.asc ? act : __builtin_bitreverse32(act) >> (32 - nu);
         const double inv_sj = invs[j];
-        const T qj = (T)(g.cone ? inv_sj * inv_sj : fabs(inv_sj));
+        const T qj = (T)(CONE ? inv_sj * inv_sj : fabs(inv_sj));
         int m = __builtin_amdgcn_readlane(m_l, b0);
         // rows m and m+1 in flight; rp: next row to read (uniform)
         const CTLAB_GMEM T* __restrict__ rp = cz + j * slab_off + (long)m * row_off;
@@ -260,7 +264,7 @@ __device__ __forceinline__ void proj_walk_wave(const DevGeom& g, const CTLAB_GME
             // lane s < nu: top boundary of column step s (its table is region s);
             // the lane test keeps the shift below 32 bits
             if (lane < nu && ((actk >> lane) & 1u)) {
-                const double t = g.cone ? sj * tau_w : 1.0;
+                const double t = CONE ? sj * tau_w : 1.0;
                 acc_top += lds_interp(G + lane * kGStride, vp.beta0 + t * ztop, ZN) * qj;
             }
         }

@@ -64,7 +64,8 @@ __attribute__((amdgpu_waves_per_eu(8))) dd_proj_walk(DevGeom g, const T* __restr
     const int grp = (int)(blockIdx.x & 7) * G8 + (int)(w % (unsigned)G8);
     const int p = grp * 4 + wv;
     if (tile >= ntiles || p >= g.PN) return;
-    proj_walk_wave<T, NU>(g, gmem(cz), gmem(prj), p, tile, threadIdx.x & 63, sG[wv], sFT[wv]);
+    if (g.cone) proj_walk_wave<T, NU, true>(g, gmem(cz), gmem(prj), p, tile, threadIdx.x & 63, sG[wv], sFT[wv]);
+    else proj_walk_wave<T, NU, false>(g, gmem(cz), gmem(prj), p, tile, threadIdx.x & 63, sG[wv], sFT[wv]);
 }
 
 // One wavefront per (view, column): v-prefix of y*L for the backprojector.
