@@ -103,6 +103,23 @@ __device__ __forceinline__ void zprefix_wave(const CTLAB_GMEM float* __restrict_
     if (lane < ZN) cz[col * ZN + lane] = r;
 }
 
+// buffer-resource word 3 for raw (untyped, unswizzled) buffer access on gfx9
+constexpr int kBufWord3 = 0x00020000;
+
+// raw buffer load of one element at byte offset off; an offset at or past the
+// resource's size reads 0
+template <typename T> __device__ __forceinline__ T buf_load(__amdgpu_buffer_rsrc_t r, unsigned off);
+template <> __device__ __forceinline__ double buf_load<double>(__amdgpu_buffer_rsrc_t r, unsigned off)
+{
+    typedef unsigned int u2 __attribute__((ext_vector_type(2)));
+    const u2 b = __builtin_amdgcn_raw_buffer_load_b64(r, off, 0, 0);
+    return __longlong_as_double(((long long)b.y << 32) | b.x);
+}
+template <> __device__ __forceinline__ float buf_load<float>(__amdgpu_buffer_rsrc_t r, unsigned off)
+{
+    return __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(r, off, 0, 0));
+}
+
 // LDS per projector wavefront: one z-prefix table (entries 0..64) per column
 // step of the tile, kGStride entries apart
 constexpr int kGStride = 68;
@@ -220,8 +237,12 @@ __device__ __forceinline__ void proj_walk_wave(const DevGeom& g, const CTLAB_GME
         int m = __builtin_amdgcn_readlane(m_l, b0);
         // rows m and m+1 in flight; rp: next row to read (uniform)
         const CTLAB_GMEM T* __restrict__ rp = cz + j * slab_off + (long)m * row_off;
-        auto row = [zoff](const CTLAB_GMEM T* r) {
-            return *(const CTLAB_GMEM T*)((const CTLAB_GMEM char*)r + zoff);
+        // a row is read through a buffer resource built from the uniform row
+        // pointer (scalar instructions only) with the lane's offset in z
+        auto row = [zoff, ZN](const CTLAB_GMEM T* r) {
+            const __amdgpu_buffer_rsrc_t rr =
+                __builtin_amdgcn_make_buffer_rsrc((void*)r, (short)0, ZN * (int)sizeof(T), kBufWord3);
+            return buf_load<T>(rr, zoff);
         };
         T c0 = row(rp);
         T c1 = row(rp + row_off);
@@ -374,23 +395,6 @@ __device__ __forceinline__ T column_delta(const DevGeom& g, T* G, T Pu, double e
 // slope NaN, 1/|slope step| and esdz 0), and one spare record follows the last
 // view.
 struct ColRec { double klo, khi, invdk, esdz; };
-
-// buffer-resource word 3 for raw (untyped, unswizzled) buffer access on gfx9
-constexpr int kBufWord3 = 0x00020000;
-
-// raw buffer load of one element at byte offset off; an offset at or past the
-// resource's size reads 0
-template <typename T> __device__ __forceinline__ T buf_load(__amdgpu_buffer_rsrc_t r, unsigned off);
-template <> __device__ __forceinline__ double buf_load<double>(__amdgpu_buffer_rsrc_t r, unsigned off)
-{
-    typedef unsigned int u2 __attribute__((ext_vector_type(2)));
-    const u2 b = __builtin_amdgcn_raw_buffer_load_b64(r, off, 0, 0);
-    return __longlong_as_double(((long long)b.y << 32) | b.x);
-}
-template <> __device__ __forceinline__ float buf_load<float>(__amdgpu_buffer_rsrc_t r, unsigned off)
-{
-    return __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(r, off, 0, 0));
-}
 
 // Backprojector walk of one (slab j, tile rt of NM rows) over the views of one
 // slab orientation; lane = z.  acc[i] (zeroed here) receives voxel row
