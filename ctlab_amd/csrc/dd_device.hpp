@@ -80,15 +80,35 @@ __device__ __forceinline__ T lds_interp(const T* G, double b, int n)
 // ---------------------------------------------------------------------------
 // z-prefix of every masked voxel column: cz[col*ZN + k] = sum_{k'<=k} f*mask
 // ---------------------------------------------------------------------------
-// inclusive prefix over the 64 lanes of a wavefront (lane l gets sum of lanes 0..l)
-template <typename T>
-__device__ __forceinline__ T wave_inclusive_scan(T v, int lane)
+// DPP move of v with control `ctrl` on the rows in row_mask; lanes without a
+// source, and rows outside the mask, receive 0
+template <int ctrl, int row_mask>
+__device__ __forceinline__ double dpp_or0(double v)
 {
-#pragma unroll
-    for (int o = 1; o < 64; o <<= 1) {
-        const T n = __shfl_up(v, o, 64);
-        if (lane >= o) v += n;
-    }
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_update_dpp(0, (int)(b & 0xffffffffLL), ctrl, row_mask, 0xf, true);
+    const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), ctrl, row_mask, 0xf, true);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+template <int ctrl, int row_mask>
+__device__ __forceinline__ float dpp_or0(float v)
+{
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), ctrl, row_mask, 0xf, true));
+}
+
+// inclusive prefix over the 64 lanes of a wavefront (lane l gets sum of lanes
+// 0..l), in DPP moves: shifts by 1, 2, 4, 8 lanes within each row of 16, then
+// the last lane of row 0 broadcast into row 1 (and of row 2 into row 3), then
+// lane 31 into rows 2 and 3
+template <typename T>
+__device__ __forceinline__ T wave_inclusive_scan(T v)
+{
+    v += dpp_or0<0x111, 0xf>(v);   // row_shr:1
+    v += dpp_or0<0x112, 0xf>(v);   // row_shr:2
+    v += dpp_or0<0x114, 0xf>(v);   // row_shr:4
+    v += dpp_or0<0x118, 0xf>(v);   // row_shr:8
+    v += dpp_or0<0x142, 0xa>(v);   // row_bcast:15 into rows 1 and 3
+    v += dpp_or0<0x143, 0xc>(v);   // row_bcast:31 into rows 2 and 3
     return v;
 }
 
@@ -99,7 +119,7 @@ __device__ __forceinline__ void zprefix_wave(const CTLAB_GMEM float* __restrict_
 {
     const T mk = (mask == nullptr || mask[col]) ? T(1) : T(0);
     const T v = lane < ZN ? (T)vol[col * ZN + lane] * mk : T(0);
-    const T r = wave_inclusive_scan(v, lane);
+    const T r = wave_inclusive_scan(v);
     if (lane < ZN) cz[col * ZN + lane] = r;
 }
 
@@ -340,7 +360,7 @@ __device__ __forceinline__ void prjprefix_wave(const DevGeom& g, const CTLAB_GME
         }
         v = (T)((double)y * L);
     }
-    const T r = wave_inclusive_scan(v, lane);
+    const T r = wave_inclusive_scan(v);
     if (lane < DNV) P[idx * DNV + lane] = r;
 }
 

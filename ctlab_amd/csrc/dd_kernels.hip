@@ -68,14 +68,39 @@ __attribute__((amdgpu_waves_per_eu(8))) dd_proj_walk(DevGeom g, const T* __restr
     else proj_walk_wave<T, NU, false>(g, gmem(cz), gmem(prj), p, tile, threadIdx.x & 63, sG[wv], sFT[wv]);
 }
 
-// One wavefront per (view, column): v-prefix of y*L for the backprojector.
+// v-prefix of y*L for the backprojector (the same values as prjprefix_wave):
+// one wavefront per kPrefixCols consecutive (view, column) rows, whose loads
+// are all issued before the first scan, so that they are in flight together.
+constexpr int kPrefixCols = 4;
 template <typename T>
 __global__ void __launch_bounds__(256)
 prjprefix_kernel(DevGeom g, const float* __restrict__ prj, T* __restrict__ P)
 {
-    const long idx = blockIdx.x * 4L + (threadIdx.x >> 6);
-    if (idx >= (long)g.PN * g.DNU) return;
-    prjprefix_wave<T>(g, gmem(prj), gmem(P), idx, threadIdx.x & 63);
+    const long idx0 = (blockIdx.x * 4L + (threadIdx.x >> 6)) * kPrefixCols;
+    const long n = (long)g.PN * g.DNU;
+    const int lane = threadIdx.x & 63, DNV = g.DNV;
+    if (idx0 >= n) return;
+    // every load of the wave's rows first (rows past the last read the last one)
+    const int vl = min(lane, DNV - 1);
+    const double ezv = g.ez[vl];
+    float y[kPrefixCols];
+    double e2[kPrefixCols], ta[kPrefixCols];
+#pragma unroll
+    for (int c = 0; c < kPrefixCols; ++c) {
+        const long idx = min(idx0 + c, n - 1);
+        y[c] = prj[idx * DNV + vl];
+        e2[c] = g.exy2[idx];
+        ta[c] = g.tau[idx];
+    }
+#pragma unroll
+    for (int c = 0; c < kPrefixCols; ++c) {
+        const long idx = idx0 + c;
+        const double L = g.dx * sqrt(e2[c] + ezv * ezv) * fabs(ta[c]);
+        const T r = wave_inclusive_scan(lane < DNV ? (T)((double)y[c] * L) : T(0));
+        if (idx < n && lane < DNV) {
+            P[idx * DNV + lane] = r;
+        }
+    }
 }
 
 // First walk-order column whose upper boundary lies above row r0 = 8*r8, per
@@ -247,7 +272,7 @@ hipError_t launch_back(const DevGeom& g, const float* d_prj, void* d_P, float* d
     const ColRec* rec = (const ColRec*)d_rec;
     T* P = static_cast<T*>(d_P);
     const long ncu = (long)g.PN * g.DNU;
-    hipLaunchKernelGGL(prjprefix_kernel<T>, dim3((unsigned)((ncu + 3) / 4)), dim3(256), 0, s,
+    hipLaunchKernelGGL(prjprefix_kernel<T>, dim3((unsigned)((ncu + 4 * kPrefixCols - 1) / (4 * kPrefixCols))), dim3(256), 0, s,
                        g, d_prj, P);
     // y-slab views (slab = y, rows along x) into d_vol, x-slab views into d_scratch, one launch
     switch (g.tile_nm) {
