@@ -922,8 +922,13 @@ def main():
                      "achieved_is": "streaming-model bytes / time (effective bandwidth)",
                      "measured_gbps": traffic_b / t_dom / 1e9 if traffic_b else None,
                      "measured_frac": traffic_b / t_dom / HBM_PEAK if traffic_b else None,
-                     "limit": "VALU issue (SQ counters: 72-82 % VALU busy at 8 wavefronts per SIMD), "
-                              "not HBM bandwidth"},
+                     # SQ counters of the walks (cfg2, tiles 8/16): the backprojector waits for an issue
+                     # slot 36 % of its wave cycles at 73 % VALU / 59 % SALU utilisation; the FP64
+                     # projector waits on dependencies 65 % at 62 % VALU
+                     "limit": ("VALU and scalar issue (73 % VALU, 59 % SALU utilisation), not HBM bandwidth"
+                               if dominant == "back" else
+                               "latency of its 8-byte row and LDS traffic (62 % VALU utilisation, 65 % of "
+                               "wave cycles in dependency waits), not HBM bandwidth")},
         "ms_proj": float(np.mean(tp)) * 1e3,
         "ms_back": float(np.mean(tb)) * 1e3,
         "wall_s": wall,
