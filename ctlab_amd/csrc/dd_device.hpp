@@ -164,12 +164,12 @@ constexpr int kGStride = 68;
 // at, magnification of the column it crosses), so that every step reads its
 // two uniform operands with one broadcast LDS read.
 //
-// Rows are read two ahead through a pointer that steps one row at a time; the
-// read-ahead runs up to two rows past the slab's last row, which the
-// integral-image buffer provides as padding (2 * (XN + 1) * ZN elements).
-// (Reading no row past the walk's last one saves L2 traffic but measured 18 %
-// slower: the branch around each read breaks the load pipelining.)  Only the
-// slabs the tile can reach are walked, an interval of slabs found once per tile.
+// Rows are read two ahead through a pointer that steps one row at a time; a
+// read-ahead past the walk's last row in the slab re-reads that row (a scalar
+// select of the address, no branch: a branch around each read measured 18 %
+// slower, it breaks the load pipelining), so the reads stay on the walk's rows
+// and its surplus read-aheads hit the cache.  Only the slabs the tile can
+// reach are walked, an interval of slabs found once per tile.
 template <typename T, int NU, bool CONE>
 __device__ __forceinline__ void proj_walk_wave(const DevGeom& g, const CTLAB_GMEM T* __restrict__ cz,
                                                CTLAB_GMEM float* __restrict__ prj,
@@ -264,8 +264,12 @@ __device__ __forceinline__ void proj_walk_wave(const DevGeom& g, const CTLAB_GME
                 __builtin_amdgcn_make_buffer_rsrc((void*)r, (short)0, ZN * (int)sizeof(T), kBufWord3);
             return buf_load<T>(rr, zoff);
         };
+        // the walk's last row in the slab: read-aheads past it re-read it
+        const int m_end = __builtin_amdgcn_readlane(m_l, bl);
+        const CTLAB_GMEM T* __restrict__ rp_end = rp + (long)(m_end - m) * row_off;
+        int mr = m + 2;   // row index of rp
         T c0 = row(rp);
-        T c1 = row(rp + row_off);
+        T c1 = row(m + 1 <= m_end ? rp + row_off : rp_end);
         rp += 2 * row_off;
         T S = T(0);
         T Eprev = (T)readlane_d(f_l, b0) * c0;
@@ -279,8 +283,9 @@ __device__ __forceinline__ void proj_walk_wave(const DevGeom& g, const CTLAB_GME
                 S += c0;
                 ++m;
                 c0 = c1;
-                c1 = row(rp);
+                c1 = row(mr <= m_end ? rp : rp_end);
                 rp += row_off;
+                ++mr;
             }
             // both step operands in one LDS read: the empty asm makes the
             // magnification, used only in the active branch, live here
