@@ -28,12 +28,11 @@ struct GeomIn {
 constexpr int kBlock = 64;
 inline bool is_blocked(const GeomIn& in) { return in.ZN > kBlock || in.DNV > kBlock; }
 
-// integral-image elements: the volume plus two rows of padding for the
-// projector's read-ahead past a slab's last row (an x-slab row is XN*ZN
-// elements apart, a y-slab row ZN)
+// integral-image elements: one per voxel (the projector's row reads stay on
+// the rows of the slab it walks)
 inline size_t cz_elems(const GeomIn& in)
 {
-    return (size_t)in.XN * in.YN * in.ZN + 2 * ((size_t)in.XN + 1) * in.ZN;
+    return (size_t)in.XN * in.YN * in.ZN;
 }
 
 
