@@ -169,6 +169,27 @@ def test_views_on_45_degree_ties(cuda):
     assert rel_l2(_run(g, vol=vol), O.proj(g, vol)) <= TOL64
 
 
+@pytest.mark.parametrize("nu,nm", [(8, 16), (12, 8)])
+def test_axis_aligned_views_tile_edge_on_central_ray(cuda, nu, nm):
+    """Views at exact multiples of 90 degrees with a centred 48-column detector:
+    at 0 degrees the central column boundary (24, a tile edge for 8- and
+    12-column tiles) has slope exactly 0, so the projector's slab interval for
+    the tiles on either side takes its zero-slope branch."""
+    import torch
+    g = _cone(XN=36, YN=40, ZN=10, PN=4, DNU=48, DNV=10, ang0=0.0)
+    g.angs = np.array([0.0, np.pi / 2, np.pi, 1.5 * np.pi], dtype=np.float32)
+    rng = np.random.default_rng(23)
+    vol = rng.random(g.vol_shape).astype(np.float32)
+    prj = rng.random(g.nprj()).astype(np.float32)
+    pl = dd.Plan(g)
+    pl.set_tiles(nu, nm)
+    p = pl.proj(torch.from_numpy(vol).cuda())
+    b = pl.back(torch.from_numpy(prj).cuda())
+    torch.cuda.synchronize()
+    assert rel_l2(p.cpu().numpy(), O.proj(g, vol)) <= TOL64
+    assert rel_l2(b.cpu().numpy(), O.back(g, prj)) <= TOL64
+
+
 @pytest.mark.parametrize("sln", [1, 3])
 def test_fan_multislice_parity(cuda, sln):
     g = G.fan_multislice(XN=64, YN=64, SLN=sln, dx=1.0, PN=60, DNU=140, col_size=1.0, sid=150.0,
