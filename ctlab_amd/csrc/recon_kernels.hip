@@ -218,33 +218,34 @@ __global__ void __launch_bounds__(256) k_tv_primal(const float* __restrict__ b, 
 
 // One FGP iteration, fused.  The extrapolated dual r = pc + mom*(pc - pp) (pc,
 // pp: the last two dual iterates, 3n floats each) is formed where it is read;
-// tmp = b - lam * L(r) is evaluated at the voxel and its three forward
-// neighbours (k_tv_primal's arithmetic); then
+// tmp = b - lam * L(r) (k_tv_primal's arithmetic) is needed at the voxel and
+// its three forward neighbours; then
 //   pn = proj_{|.|<=1}(r + step * L^T(tmp))
 // goes to a third buffer, so no thread writes what another one reads.  Per
 // iteration this moves 9 floats per voxel through HBM (pc, pp, b, pn) where a
 // primal and a dual pass over stored r and tmp moved 17.
+// A block covers 256 consecutive points of one y-plane (z fastest, then x):
+// it forms tmp once at each of them and at the ZN points after them (the x + 1
+// neighbours of its last row) into LDS, and reads the x + 1 and z + 1
+// neighbours' tmp from there; only the y + 1 neighbour's is formed per thread.
+// Launched with (256 + ZN) floats of dynamic LDS.
 __global__ void __launch_bounds__(256) k_tv_fgp(const float* __restrict__ b, const float* __restrict__ pc,
                                                 const float* __restrict__ pp, float mom, float lam, float step,
                                                 int XN, int YN, int ZN, float rcpZN, float* __restrict__ pn)
 {
+    extern __shared__ float T[];
     const long n = (long)XN * YN * ZN;
-    const RowIdx q = row_index(XN, ZN, rcpZN);
-    if (!q.ok) return;
-    const long i = q.i;
     const int y = blockIdx.y;
     const long ex = ZN, ey = (long)XN * ZN;
+    const int nplane = XN * ZN;
+    const int j0 = blockIdx.x * 256;
     auto r = [&](int c, long j) {
         const float a = pc[c * n + j];
         return a + mom * (a - pp[c * n + j]);
     };
     // Every load is unconditional: a neighbour that does not exist is read at
-    // index i (valid) and its term masked by a select, so all 46 loads are in
-    // flight together (behind branches they came in four or five round trips).
-    // Subtracting an exact 0 leaves each sum bit for bit as k_tv_primal forms it.
-    const bool lx = q.x > 0, ly = y > 0, lz = q.z > 0;                     // backward neighbours
-    const bool hx = q.x < XN - 1, hy = y < YN - 1, hz = q.z < ZN - 1;      // forward neighbours
-    auto nb = [&](bool ok, long j) { return ok ? j : i; };
+    // a valid index and its term masked by a select.  Subtracting an exact 0
+    // leaves each sum bit for bit as k_tv_primal forms it.
     auto tmp_at = [&](long j, bool mx, long jx, bool my, long jy, bool mz, long jz) {
         float v = r(0, j) + r(1, j) + r(2, j);
         const float sx = r(0, jx), sy = r(1, jy), sz = r(2, jz);
@@ -253,11 +254,30 @@ __global__ void __launch_bounds__(256) k_tv_fgp(const float* __restrict__ b, con
         v -= mz ? sz : 0.0f;
         return b[j] - lam * v;
     };
-    const float t = tmp_at(i, lx, nb(lx, i - ex), ly, nb(ly, i - ey), lz, nb(lz, i - 1));
-    const long jX = nb(hx, i + ex), jY = nb(hy, i + ey), jZ = nb(hz, i + 1);
-    const float tX = tmp_at(jX, true, i, ly, nb(hx && ly, i + ex - ey), lz, nb(hx && lz, i + ex - 1));
+    const bool ly = y > 0, hy = y < YN - 1;
+    for (int k = threadIdx.x; k < 256 + ZN; k += 256) {
+        const int j = j0 + k;
+        if (j < nplane) {
+            int xq = (int)((float)j * rcpZN);
+            int zq = j - xq * ZN;
+            if (zq < 0) { --xq; zq += ZN; } else if (zq >= ZN) { ++xq; zq -= ZN; }
+            const long i = (long)y * ey + j;
+            const bool mx = xq > 0, mz = zq > 0;
+            T[k] = tmp_at(i, mx, mx ? i - ex : i, ly, ly ? i - ey : i, mz, mz ? i - 1 : i);
+        }
+    }
+    __syncthreads();
+    const RowIdx q = row_index(XN, ZN, rcpZN);
+    if (!q.ok) return;
+    const long i = q.i;
+    const bool lx = q.x > 0, lz = q.z > 0;                               // backward neighbours
+    const bool hx = q.x < XN - 1, hz = q.z < ZN - 1;                     // forward neighbours
+    auto nb = [&](bool ok, long j) { return ok ? j : i; };
+    const float t = T[threadIdx.x];
+    const float tX = T[threadIdx.x + ZN];   // read only where hx (inside the plane)
+    const float tZ = T[threadIdx.x + 1];    // read only where hz (same x row)
+    const long jY = nb(hy, i + ey);
     const float tY = tmp_at(jY, lx, nb(hy && lx, i + ey - ex), true, i, lz, nb(hy && lz, i + ey - 1));
-    const float tZ = tmp_at(jZ, lx, nb(hz && lx, i + 1 - ex), ly, nb(hz && ly, i + 1 - ey), true, i);
     const float qx = hx ? t - tX : 0.0f;
     const float qy = hy ? t - tY : 0.0f;
     const float qz = hz ? t - tZ : 0.0f;
@@ -353,7 +373,8 @@ hipError_t rk_tv_prox(const float* b, float* x, float lam, int iters, int XN, in
     if (lam <= 0.0f) {
         return hipMemcpyAsync(x, b, n * sizeof(float), hipMemcpyDeviceToDevice, s);
     }
-    if ((long)XN * ZN > (1L << 22) || YN > 65535) return hipErrorInvalidValue;   // row index and grid limits
+    // row index and grid limits; the FGP kernel's (256 + ZN) floats of LDS within 64 KiB
+    if ((long)XN * ZN > (1L << 22) || YN > 65535 || ZN > 16128) return hipErrorInvalidValue;
     const dim3 grid((unsigned)(((long)XN * ZN + 255) / 256), (unsigned)YN);
     const float rcp = 1.0f / (float)ZN;
     const int nd = (XN > 1) + (YN > 1) + (ZN > 1);
@@ -366,8 +387,8 @@ hipError_t rk_tv_prox(const float* b, float* x, float lam, int iters, int XN, in
     double t = 1.0;
     for (int it = 0; it < iters; ++it) {
         const int nxt = 3 - cur - prev;
-        hipLaunchKernelGGL(k_tv_fgp, grid, dim3(256), 0, s, b, P[cur], P[prev], mom, lam, step, XN, YN, ZN, rcp,
-                           P[nxt]);
+        hipLaunchKernelGGL(k_tv_fgp, grid, dim3(256), (256 + ZN) * sizeof(float), s, b, P[cur], P[prev], mom, lam,
+                           step, XN, YN, ZN, rcp, P[nxt]);
         const double tn = 0.5 * (1.0 + __builtin_sqrt(1.0 + 4.0 * t * t));
         mom = (float)((t - 1.0) / tn);                  // extrapolates from this iteration's output
         t = tn;
