@@ -229,6 +229,12 @@ __global__ void __launch_bounds__(256) k_tv_primal(const float* __restrict__ b, 
 // neighbours of its last row) into LDS, and reads the x + 1 and z + 1
 // neighbours' tmp from there; only the y + 1 neighbour's is formed per thread.
 // Launched with (256 + ZN) floats of dynamic LDS.
+// R: kDualAll reads pc and pp; kDualNoMom (momentum exactly 0, the second
+// iteration) reads pc only, r = pc; kDualZero (the first iteration, from
+// p = 0) reads neither, r = 0.  Each equals kDualAll on the zero-filled
+// buffers bit for bit, without the fill.
+enum { kDualAll = 0, kDualNoMom = 1, kDualZero = 2 };
+template <int R>
 __global__ void __launch_bounds__(256) k_tv_fgp(const float* __restrict__ b, const float* __restrict__ pc,
                                                 const float* __restrict__ pp, float mom, float lam, float step,
                                                 int XN, int YN, int ZN, float rcpZN, float* __restrict__ pn)
@@ -240,7 +246,9 @@ __global__ void __launch_bounds__(256) k_tv_fgp(const float* __restrict__ b, con
     const int nplane = XN * ZN;
     const int j0 = blockIdx.x * 256;
     auto r = [&](int c, long j) {
+        if constexpr (R == kDualZero) return 0.0f;
         const float a = pc[c * n + j];
+        if constexpr (R == kDualNoMom) return a;
         return a + mom * (a - pp[c * n + j]);
     };
     // Every load is unconditional: a neighbour that does not exist is read at
@@ -380,15 +388,25 @@ hipError_t rk_tv_prox(const float* b, float* x, float lam, int iters, int XN, in
     const int nd = (XN > 1) + (YN > 1) + (ZN > 1);
     const float step = 1.0f / (4.0f * (float)(nd > 0 ? nd : 1) * lam);
     float* P[3] = {work, work + 3 * n, work + 6 * n};
-    // p_0 = 0 and the iterate before it = 0: the first r is 0
-    hipLaunchKernelGGL(k_fill, dim3(ew_grid(6 * n)), dim3(kEwBlock), 0, s, work, 0.0f, 6 * n);
+    // p_0 = 0 and the iterate before it = 0: the first r is 0, which the first
+    // iteration uses without reading the buffers; with no iteration the
+    // primal reads p_0 itself
+    if (iters <= 0) hipLaunchKernelGGL(k_fill, dim3(ew_grid(3 * n)), dim3(kEwBlock), 0, s, work, 0.0f, 3 * n);
     int cur = 0, prev = 1;
     float mom = 0.0f;
     double t = 1.0;
+    const size_t lds = (256 + ZN) * sizeof(float);
     for (int it = 0; it < iters; ++it) {
         const int nxt = 3 - cur - prev;
-        hipLaunchKernelGGL(k_tv_fgp, grid, dim3(256), (256 + ZN) * sizeof(float), s, b, P[cur], P[prev], mom, lam,
-                           step, XN, YN, ZN, rcp, P[nxt]);
+        if (it == 0)
+            hipLaunchKernelGGL(k_tv_fgp<kDualZero>, grid, dim3(256), lds, s, b, P[cur], P[prev], mom, lam, step, XN,
+                               YN, ZN, rcp, P[nxt]);
+        else if (mom == 0.0f)
+            hipLaunchKernelGGL(k_tv_fgp<kDualNoMom>, grid, dim3(256), lds, s, b, P[cur], P[prev], mom, lam, step, XN,
+                               YN, ZN, rcp, P[nxt]);
+        else
+            hipLaunchKernelGGL(k_tv_fgp<kDualAll>, grid, dim3(256), lds, s, b, P[cur], P[prev], mom, lam, step, XN,
+                               YN, ZN, rcp, P[nxt]);
         const double tn = 0.5 * (1.0 + __builtin_sqrt(1.0 + 4.0 * t * t));
         mom = (float)((t - 1.0) / tn);                  // extrapolates from this iteration's output
         t = tn;
