@@ -165,11 +165,12 @@ constexpr int kGStride = 68;
 // two uniform operands with one broadcast LDS read.
 //
 // Rows are read two ahead through a pointer that steps one row at a time; a
-// read-ahead past the walk's last row in the slab re-reads that row (a scalar
-// select of the address, no branch: a branch around each read measured 18 %
-// slower, it breaks the load pipelining), so the reads stay on the walk's rows
-// and its surplus read-aheads hit the cache.  Only the slabs the tile can
-// reach are walked, an interval of slabs found once per tile.
+// read-ahead past the walk's last row in the slab is issued with a buffer
+// resource of size 0 (a scalar select of its size, no branch: a branch around
+// each read measured 18 % slower, it breaks the load pipelining), so it moves
+// no data: re-reading the last row instead cost 2 % of the projector time in
+// L1 traffic.  Only the slabs the tile can reach are walked, an interval of
+// slabs found once per tile.
 template <typename T, int NU, bool CONE>
 __device__ __forceinline__ void proj_walk_wave(const DevGeom& g, const CTLAB_GMEM T* __restrict__ cz,
                                                CTLAB_GMEM float* __restrict__ prj,
@@ -258,18 +259,18 @@ __device__ __forceinline__ void proj_walk_wave(const DevGeom& g, const CTLAB_GME
         // rows m and m+1 in flight; rp: next row to read (uniform)
         const CTLAB_GMEM T* __restrict__ rp = cz + j * slab_off + (long)m * row_off;
         // a row is read through a buffer resource built from the uniform row
-        // pointer (scalar instructions only) with the lane's offset in z
-        auto row = [zoff, ZN](const CTLAB_GMEM T* r) {
+        // pointer (scalar instructions only) with the lane's offset in z; a
+        // read-ahead past the walk's last row in the slab gets a resource of
+        // size 0, which reads nothing (and returns 0, never used)
+        auto row = [zoff, ZN](const CTLAB_GMEM T* r, bool live) {
             const __amdgpu_buffer_rsrc_t rr =
-                __builtin_amdgcn_make_buffer_rsrc((void*)r, (short)0, ZN * (int)sizeof(T), kBufWord3);
+                __builtin_amdgcn_make_buffer_rsrc((void*)r, (short)0, live ? ZN * (int)sizeof(T) : 0, kBufWord3);
             return buf_load<T>(rr, zoff);
         };
-        // the walk's last row in the slab: read-aheads past it re-read it
-        const int m_end = __builtin_amdgcn_readlane(m_l, bl);
-        const CTLAB_GMEM T* __restrict__ rp_end = rp + (long)(m_end - m) * row_off;
+        const int m_end = __builtin_amdgcn_readlane(m_l, bl);   // the walk's last row in the slab
         int mr = m + 2;   // row index of rp
-        T c0 = row(rp);
-        T c1 = row(m + 1 <= m_end ? rp + row_off : rp_end);
+        T c0 = row(rp, true);
+        T c1 = row(rp + row_off, m + 1 <= m_end);
         rp += 2 * row_off;
         T S = T(0);
         T Eprev = (T)readlane_d(f_l, b0) * c0;
@@ -283,7 +284,7 @@ __device__ __forceinline__ void proj_walk_wave(const DevGeom& g, const CTLAB_GME
                 S += c0;
                 ++m;
                 c0 = c1;
-                c1 = row(mr <= m_end ? rp : rp_end);
+                c1 = row(rp, mr <= m_end);
                 rp += row_off;
                 ++mr;
             }
