@@ -25,6 +25,7 @@
 // wavefront owns a tile of voxels of one slab and loops over all views of one
 // slab orientation, so no atomics are needed.
 #include "dd_device.hpp"
+#include "plan.hpp"
 
 #include <algorithm>
 #include <type_traits>
@@ -243,6 +244,7 @@ static void launch_back_walk(const DevGeom& g, const T* P, float* d_vol, float* 
 template <typename T>
 hipError_t launch_proj(const DevGeom& g, const float* d_vol, void* d_cz, float* d_prj, hipStream_t s)
 {
+    if (is_fan2d(g)) return launch_fan2d_proj<T>(g, d_vol, d_cz, d_prj, s);
     T* cz = static_cast<T*>(d_cz);
     const long ncol = (long)g.XN * g.YN;
     hipLaunchKernelGGL(zprefix_kernel<T>, dim3((unsigned)((ncol + 3) / 4)), dim3(256), 0, s,
@@ -274,6 +276,8 @@ hipError_t launch_back(const DevGeom& g, const float* d_prj, void* d_P, float* d
     const long ncu = (long)g.PN * g.DNU;
     hipLaunchKernelGGL(prjprefix_kernel<T>, dim3((unsigned)((ncu + 4 * kPrefixCols - 1) / (4 * kPrefixCols))), dim3(256), 0, s,
                        g, d_prj, P);
+    if (is_fan2d(g))
+        return launch_fan2d_back<T>(g, P, d_vol, d_scratch, d_views_y, ny, d_views_x, nx, d_rec, d_start, SJ, SR, s);
     // y-slab views (slab = y, rows along x) into d_vol, x-slab views into d_scratch, one launch
     switch (g.tile_nm) {
     case 8: launch_back_walk<T, 8>(g, P, d_vol, d_scratch, d_views_y, ny, d_views_x, nx, rec, d_start, SJ, SR, s); break;

@@ -2,6 +2,7 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <vector>
 
 #include "dd_common.hpp"
@@ -29,11 +30,24 @@ constexpr int kBlock = 64;
 inline bool is_blocked(const GeomIn& in) { return in.ZN > kBlock || in.DNV > kBlock; }
 
 // integral-image elements: one per voxel (the projector's row reads stay on
-// the rows of the slab it walks)
+// the rows of the slab it walks); a single slice also holds the row prefixes
+// of both slab orientations (YN * (XN + 1) + XN * (YN + 1) entries) that the
+// single-slice fan-beam projector reads
 inline size_t cz_elems(const GeomIn& in)
 {
-    return (size_t)in.XN * in.YN * in.ZN;
+    const size_t vox = (size_t)in.XN * in.YN * in.ZN;
+    if (in.ZN != 1) return vox;
+    return std::max(vox, (size_t)in.YN * (in.XN + 1) + (size_t)in.XN * (in.YN + 1));
 }
+
+// single-slice fan beam (ZN = DNV = 1): lanes over boundaries and views
+bool is_fan2d(const DevGeom& g);
+template <typename T>
+hipError_t launch_fan2d_proj(const DevGeom& g, const float* d_vol, void* d_cz, float* d_prj, hipStream_t s);
+template <typename T>
+hipError_t launch_fan2d_back(const DevGeom& g, const T* P, float* d_vol, float* d_scratch, const int* d_views_y,
+                             int ny, const int* d_views_x, int nx, const void* d_rec, const unsigned short* d_start,
+                             int SJ, int SR, hipStream_t s);
 
 
 template <typename T>
