@@ -9,9 +9,9 @@
 //     m voxels, piecewise linear in between) is built once per call, in both
 //     slab orientations.  Column u's footprint integral on slab j is
 //     C(a_{u+1}) - C(a_u) at its two boundaries' row coordinates a (clamped
-//     to the slab), which is what the walk's running row sums give; a
-//     wavefront holds 64 boundaries of one view (63 columns), each lane reads
-//     C at its own boundary and takes its upper neighbour's by a DPP shift.
+//     to the slab), which is what the walk's running row sums give; a block
+//     holds 64 boundaries of one view (63 columns), each lane reads C at its
+//     own boundary and takes its upper neighbour's by a DPP shift.
 //   backprojector: the walk of back_walk_accum (column records in slope
 //     space, start table, running column sums W at the row boundaries) with
 //     lane = view: a wavefront (slab, tile of NM rows) walks 64 views of one
@@ -59,8 +59,11 @@ __global__ void __launch_bounds__(256) fan2d_prefix_kernel(DevGeom g, const floa
     }
 }
 
-// Projector: wavefront (view p, tile of 63 columns from u0), lane l holds
-// boundary u0 + l.  The output factor is the general epilogue's (one row).
+// Projector: block (view p, tile of 63 columns from u0), lane l holds
+// boundary u0 + l; the block's four wavefronts take every fourth slab (a
+// wavefront's reads are a chain of slabs, so four chains finish sooner, and a
+// 36-view subset still fills the GPU), and their sums are added in LDS in a
+// fixed order.  The output factor is the general epilogue's (one row).
 constexpr int kFanCols = 63;
 
 template <typename T>
@@ -68,10 +71,10 @@ __global__ void __launch_bounds__(256) fan2d_proj_kernel(DevGeom g, const T* __r
                                                          const T* __restrict__ Px, float* __restrict__ prj,
                                                          int ntiles)
 {
-    const int w = __builtin_amdgcn_readfirstlane(blockIdx.x * 4 + (threadIdx.x >> 6));
+    __shared__ double sacc[4][64];
+    const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int lane = threadIdx.x & 63;
-    if (w >= g.PN * ntiles) return;
-    const int p = w / ntiles, tile = w - p * ntiles;
+    const int p = blockIdx.x / ntiles, tile = blockIdx.x - p * ntiles;
     const int DNU = g.DNU;
     const int u0 = tile * kFanCols, nu = min(kFanCols, DNU - u0);
     const ViewParams vp = g.vp[p];
@@ -79,10 +82,12 @@ __global__ void __launch_bounds__(256) fan2d_proj_kernel(DevGeom g, const T* __r
     const int NR = vp.NR, NS = vp.NS;
     const T* __restrict__ C = vp.yslab ? Py : Px;
     const double* __restrict__ invs = g.invs + (long)p * g.sjs;
-    double a = vp.rbase + (vp.s0 * (1.0 / g.dx)) * kap;   // the boundary's row coordinate at slab 0
+    // the boundary's row coordinate at slab wv, then four slabs per step
+    double a = vp.rbase + (vp.s0 * (1.0 / g.dx) + wv) * kap;
+    const double kap4 = 4.0 * kap;
     double acc = 0.0;
 #pragma unroll 4
-    for (int j = 0; j < NS; ++j) {
+    for (int j = wv; j < NS; j += 4) {
         const double al = fmin(fmax(a, 0.0), (double)NR);
         const int m = min((int)al, NR - 1);
         const T fr = (T)(al - (double)m);
@@ -92,8 +97,12 @@ __global__ void __launch_bounds__(256) fan2d_proj_kernel(DevGeom g, const T* __r
         const T Cn = shift_down1(Cb);                 // boundary l + 1 (lane 63's is not used)
         const T H = vp.asc ? Cn - Cb : Cb - Cn;       // footprint integral of column l
         acc += (double)H * fabs(invs[j]);
-        a += kap;
+        a += kap4;
     }
+    sacc[wv][lane] = acc;
+    __syncthreads();
+    if (wv != 0) return;
+    acc = ((sacc[0][lane] + sacc[1][lane]) + sacc[2][lane]) + sacc[3][lane];
     if (lane < nu) {
         const long cu = (long)p * DNU + u0 + lane;
         const double ez = g.ez[0], rf = g.rowfac[0];
@@ -216,9 +225,8 @@ hipError_t launch_fan2d_proj(const DevGeom& g, const float* d_vol, void* d_cz, f
     hipLaunchKernelGGL(fan2d_prefix_kernel<T>, dim3((unsigned)((g.XN + g.YN + 3) / 4)), dim3(256), 0, s, g, d_vol,
                        Py, Px);
     const int ntiles = (g.DNU + kFanCols - 1) / kFanCols;
-    const long nw = (long)g.PN * ntiles;
-    hipLaunchKernelGGL(fan2d_proj_kernel<T>, dim3((unsigned)((nw + 3) / 4)), dim3(256), 0, s, g, Py, Px, d_prj,
-                       ntiles);
+    hipLaunchKernelGGL(fan2d_proj_kernel<T>, dim3((unsigned)g.PN * (unsigned)ntiles), dim3(256), 0, s, g, Py, Px,
+                       d_prj, ntiles);
     return hipGetLastError();
 }
 
