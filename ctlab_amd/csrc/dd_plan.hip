@@ -744,12 +744,24 @@ struct CacheEntry {
     float* dv = nullptr;
     float* dp = nullptr;
     hipStream_t s = nullptr;
+    // forward calls: the view set in consecutive chunks, each a plan on the
+    // entry plan's workspace, so that a chunk's projections go back to the
+    // host (on s2) while the next chunks are projected (on s)
+    std::vector<ctlab_plan*> chunk;
+    std::vector<int> chunk_p0;
+    std::vector<hipEvent_t> chunk_done;
+    hipStream_t s2 = nullptr;
+    bool chunk_tried = false;
     std::mutex busy;
     ~CacheEntry()
     {
         if (!plan) return;
         DeviceGuard dg(device);
         if (s) hipStreamSynchronize(s);
+        if (s2) hipStreamSynchronize(s2);
+        for (ctlab_plan* c : chunk) free_plan(c);
+        for (hipEvent_t e : chunk_done) hipEventDestroy(e);
+        if (s2) hipStreamDestroy(s2);
         if (dv) hipFree(dv);
         if (dp) hipFree(dp);
         if (s) hipStreamDestroy(s);
@@ -834,8 +846,60 @@ static std::shared_ptr<CacheEntry> cache_get(std::vector<unsigned char>&& key, i
     return e;
 }
 
+// plan over the consecutive views [p0, p1) of P, borrowing P's workspace
+static ctlab_plan* view_range_plan(const ctlab_plan* P, int p0, int p1)
+{
+    ctlab_plan* S = new ctlab_plan();
+    S->in = P->in;
+    S->in.PN = p1 - p0;
+    S->in.angs.assign(P->in.angs.begin() + p0, P->in.angs.begin() + p1);
+    S->in.zpos.assign(P->in.zpos.begin() + p0, P->in.zpos.begin() + p1);
+    S->sat_bits = P->sat_bits;
+    S->tile_nu = P->tile_nu;
+    S->tile_nm = P->tile_nm;
+    return finish_plan(S, P->device, P);
+}
+
+// Forward chunks of a cached entry: 4 chunks from 256 views, 2 from 128, none
+// below or for blocked plans.  Each view's projection depends on that view
+// only, so the chunked result equals the one-launch result bit for bit.  A
+// failure leaves the entry on one launch and clears the error it set.
+static void build_chunks(CacheEntry& E)
+{
+    E.chunk_tried = true;
+    const ctlab_plan* P = E.plan;
+    const int PN = P->in.PN;
+    const int K = is_blocked(P->in) ? 1 : PN >= 256 ? 4 : PN >= 128 ? 2 : 1;
+    if (K < 2) return;
+    bool ok = !hip_check(hipStreamCreateWithFlags(&E.s2, hipStreamNonBlocking), "hipStreamCreate");
+    for (int k = 0; ok && k < K; ++k) {
+        const int p0 = (int)((long)PN * k / K), p1 = (int)((long)PN * (k + 1) / K);
+        ctlab_plan* c = view_range_plan(P, p0, p1);
+        hipEvent_t e = nullptr;
+        ok = c != nullptr && !hip_check(hipEventCreateWithFlags(&e, hipEventDisableTiming), "hipEventCreate");
+        if (c) E.chunk.push_back(c);
+        if (e) E.chunk_done.push_back(e);
+        E.chunk_p0.push_back(p0);
+    }
+    if (ok) {
+        E.chunk_p0.push_back(PN);
+        return;
+    }
+    for (ctlab_plan* c : E.chunk) free_plan(c);
+    for (hipEvent_t e : E.chunk_done) hipEventDestroy(e);
+    if (E.s2) hipStreamDestroy(E.s2);
+    E.chunk.clear();
+    E.chunk_done.clear();
+    E.chunk_p0.clear();
+    E.s2 = nullptr;
+    clear_error();
+}
+
 // host-pointer round trip used by the CTLAB entry points, on a cached entry:
-// H2D of the input, the operator, D2H of the output, all on the entry's stream
+// H2D of the input, the operator, D2H of the output.  Forward calls with
+// chunks project chunk k + 2 while chunk k's projections are copied back (a
+// copy from device memory into pageable host memory can hold the calling
+// thread until it completes, so two chunks are queued ahead of each copy).
 static void host_call(CacheEntry& E, float* hvol, float* hprj, bool forward)
 {
     ctlab_plan* P = E.plan;
@@ -843,6 +907,28 @@ static void host_call(CacheEntry& E, float* hvol, float* hprj, bool forward)
     const size_t nvox = (size_t)P->in.XN * P->in.YN * P->in.ZN;
     const size_t nprj = (size_t)P->in.PN * P->in.DNU * P->in.DNV;
     bool ok;
+    if (forward && !E.chunk_tried) build_chunks(E);
+    if (forward && !E.chunk.empty()) {
+        const int K = (int)E.chunk.size();
+        const size_t pv = (size_t)P->in.DNU * P->in.DNV;
+        auto d2h = [&](int k) {
+            const size_t o = (size_t)E.chunk_p0[k] * pv, n = (size_t)(E.chunk_p0[k + 1] - E.chunk_p0[k]) * pv;
+            return !hip_check(hipStreamWaitEvent(E.s2, E.chunk_done[k], 0), "stream wait") &&
+                   !hip_check(hipMemcpyAsync(hprj + o, E.dp + o, n * sizeof(float), hipMemcpyDeviceToHost, E.s2),
+                              "D2H");
+        };
+        ok = !hip_check(hipMemcpyAsync(E.dv, hvol, nvox * sizeof(float), hipMemcpyHostToDevice, E.s), "H2D");
+        for (int k = 0; ok && k < K; ++k) {
+            ok = plan_proj(E.chunk[k], E.dv, E.dp + (size_t)E.chunk_p0[k] * pv, E.s) == CTLAB_OK &&
+                 !hip_check(hipEventRecord(E.chunk_done[k], E.s), "event record");
+            if (ok && k >= 2) ok = d2h(k - 2);
+        }
+        for (int k = std::max(K - 2, 0); ok && k < K; ++k) ok = d2h(k);
+        hipStreamSynchronize(E.s);
+        if (ok) hip_check(hipStreamSynchronize(E.s2), "kernel execution");
+        else hipStreamSynchronize(E.s2);
+        return;
+    }
     if (forward) {
         ok = !hip_check(hipMemcpyAsync(E.dv, hvol, nvox * sizeof(float), hipMemcpyHostToDevice, E.s), "H2D") &&
              plan_proj(P, E.dv, E.dp, E.s) == CTLAB_OK &&

@@ -269,6 +269,20 @@ def test_host_abi_equals_plan(cuda):
     assert np.array_equal(dd.DD3Back_gpu(g, prj).reshape(-1), _run(g, prj=prj).reshape(-1))
 
 
+@pytest.mark.parametrize("pn", [130, 301])
+def test_host_abi_chunked_forward_equals_plan(cuda, pn):
+    """DD3Proj_gpu with 128 views or more projects the view set in chunks whose
+    copies back to the host overlap the next chunks (2 chunks from 128 views,
+    4 from 256, uneven here); every view's projection depends on that view
+    only, so the result equals one launch bit for bit, on the call that
+    builds the chunks and on the cached one after it."""
+    g = _cone(XN=32, YN=28, ZN=8, DNU=40, DNV=8, PN=pn)
+    vol = _phantom(g, 21)
+    ref = _run(g, vol=vol).reshape(-1)
+    for _ in range(2):
+        assert np.array_equal(dd.DD3Proj_gpu(g, vol).reshape(-1), ref)
+
+
 def test_gpu_adjoint_identity(cuda):
     g = _cone(PN=64)
     rng = np.random.default_rng(20)
