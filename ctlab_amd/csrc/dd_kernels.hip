@@ -276,8 +276,6 @@ hipError_t launch_back(const DevGeom& g, const float* d_prj, void* d_P, float* d
     const long ncu = (long)g.PN * g.DNU;
     hipLaunchKernelGGL(prjprefix_kernel<T>, dim3((unsigned)((ncu + 4 * kPrefixCols - 1) / (4 * kPrefixCols))), dim3(256), 0, s,
                        g, d_prj, P);
-    if (is_fan2d(g))
-        return launch_fan2d_back<T>(g, P, d_vol, d_scratch, d_views_y, ny, d_views_x, nx, d_rec, d_start, SJ, SR, s);
     // y-slab views (slab = y, rows along x) into d_vol, x-slab views into d_scratch, one launch
     switch (g.tile_nm) {
     case 8: launch_back_walk<T, 8>(g, P, d_vol, d_scratch, d_views_y, ny, d_views_x, nx, rec, d_start, SJ, SR, s); break;

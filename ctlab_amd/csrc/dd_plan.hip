@@ -301,7 +301,7 @@ void free_plan(ctlab_plan* P)
     if (P->d_sub_prj) hipFree(P->d_sub_prj);
     if (P->ws_shared) P->d_cz = P->d_P = nullptr;
     void* ptrs[] = {P->d_vp, P->d_kap, P->d_tau, P->d_colfac, P->d_exy2, P->d_invdk, P->d_esdz, P->d_invs, P->d_zhat, P->d_ez, P->d_rec, P->d_start,
-                    P->d_rowfac, P->d_mask, P->d_views, P->d_cz, P->d_P};
+                    P->d_start1, P->d_rowfac, P->d_mask, P->d_views, P->d_cz, P->d_P};
     for (void* q : ptrs)
         if (q) hipFree(q);
     hipSetDevice(cur);
@@ -318,6 +318,7 @@ static void ws_need(const GeomIn& in, size_t& ncz, size_t& nP)
     g.ZN = std::min(in.ZN, kBlock);
     ncz = cz_elems(g);
     nP = (size_t)in.PN * in.DNU * std::min(in.DNV, kBlock);
+    if (is_fan2d(in)) nP = std::max(nP, (size_t)in.PN * (in.DNU + 1));   // the columns' prefix per view
 }
 
 // (Re)allocates the integral-image and v-prefix workspaces for `bits`.  Both
@@ -402,6 +403,18 @@ ctlab_plan* finish_plan(ctlab_plan* P, int gpunum, const ctlab_plan* share_ws)
         hip_check(hipDeviceSynchronize(), "start table")) {
         free_plan(P);
         return nullptr;
+    }
+    // single-slice fan beam: the one-row start table of its backprojector, up
+    // to 1 GiB (above that the backprojector takes the general walk)
+    const size_t ns1 = (size_t)P->in.PN * P->SJ * (P->SJ + 1);
+    if (is_fan2d(P->in) && ns1 * sizeof(unsigned short) <= ((size_t)1 << 30)) {
+        P->SR1 = P->SJ + 1;
+        if (hip_check(hipMalloc((void**)&P->d_start1, ns1 * sizeof(unsigned short)), "hipMalloc(row start table)") ||
+            hip_check(launch_fan2d_start(P->dev(), P->d_rec, P->d_start1, P->SJ, P->SR1, nullptr), "row start table") ||
+            hip_check(hipDeviceSynchronize(), "row start table")) {
+            free_plan(P);
+            return nullptr;
+        }
     }
     P->rec.clear();
     P->rec.shrink_to_fit();
@@ -716,6 +729,15 @@ int plan_back(ctlab_plan* P, const float* d_prj, float* d_vol, hipStream_t s)
     const int* vx = P->d_views + ny;
     // the integral-image workspace (>= XN*YN*ZN floats) is idle during backprojection
     float* scratch = static_cast<float*>(P->d_cz);
+    if (P->d_start1) {
+        hipError_t e = P->sat_bits == 32
+                           ? launch_fan2d_back<float>(g, d_prj, P->d_P, d_vol, scratch, vy, ny, vx, nx,
+                                                      P->d_start1, P->SJ, P->SR1, s)
+                           : launch_fan2d_back<double>(g, d_prj, P->d_P, d_vol, scratch, vy, ny, vx, nx,
+                                                       P->d_start1, P->SJ, P->SR1, s);
+        if (hip_check(e, "backprojector launch")) return t_err;
+        return CTLAB_OK;
+    }
     hipError_t e = P->sat_bits == 32
                        ? launch_back<float>(g, d_prj, P->d_P, d_vol, scratch, vy, ny, vx, nx, P->d_rec, P->d_start,
                                             P->SJ, P->SR, s)

@@ -1,6 +1,6 @@
 // Single-slice fan beam (DD2_multiGPU with one slice, src/multiSlices_ker.cuh:
 // 35-38, methods 0 and 2): the branchless DD projector and backprojector with
-// lanes over detector boundaries and over views.
+// lanes over detector boundaries and over row boundaries.
 //
 // The general walks (dd_device.hpp) hold lane = z, so with one slice they
 // use one lane in 64.  Here the same quantities are formed with the lanes
@@ -12,11 +12,11 @@
 //     to the slab), which is what the walk's running row sums give; a block
 //     holds 64 boundaries of one view (63 columns), each lane reads C at its
 //     own boundary and takes its upper neighbour's by a DPP shift.
-//   backprojector: the walk of back_walk_accum (column records in slope
-//     space, start table, running column sums W at the row boundaries) with
-//     lane = view: a wavefront (slab, tile of NM rows) walks 64 views of one
-//     slab orientation at once, each lane its own view, and the rows' sums
-//     over the views are reduced across lanes at the end in a fixed order.
+//   backprojector: the walk's running column sum W at a row boundary is an
+//     interpolation of the columns' prefix sum Q at the column that holds
+//     the boundary (from a one-row start table), so a wavefront holds 64 row
+//     boundaries of one slab, reads per view only the columns its boundaries
+//     fall in, and differences W with its upper neighbour by a DPP shift.
 // Factors, clamping and the slab sums are those of the general walks, so the
 // two agree to rounding.
 #include "dd_device.hpp"
@@ -110,103 +110,133 @@ __global__ void __launch_bounds__(256) fan2d_proj_kernel(DevGeom g, const T* __r
     }
 }
 
-// Backprojector: block = 4 wavefronts = 4 consecutive slabs of one tile of
-// kFanRows rows; the first nby blocks take the y-slab views and write vol_y,
-// the rest the x-slab views into vol_x (summed by back_add), as dd_back_walk.
-constexpr int kFanRows = 16;
+// Backprojector.  For a view and slab the walk's running column sum W at row
+// boundary r (row-axis coordinate) is, with t the walk-order column that
+// holds r, W(r) = Q[t] + fr * (Q[t + 1] - Q[t]): Q[c] is the sum of the
+// column entries (y * L with one detector row) of walk-order columns below c,
+// and fr = clamp((km - klo_t) / |slope step_t|, 0, 1) with km = (r - rbase) /
+// |rs| in slope space, the walk's own fraction.  Voxel row m of the slab gets
+// W(m + 1) - W(m).  So every lane of a wavefront (slab, tile of 63 rows) holds
+// one row boundary and, per view, reads only the column that holds it (from a
+// start table with one entry per row boundary, built once per plan) and two
+// prefix entries, instead of walking every column the rows cross; its
+// upper neighbour's W comes by a DPP shift.  Block = one (slab, tile); its
+// four wavefronts take every fourth view of the list, and their sums are added
+// in LDS in a fixed order.  The first nby blocks take the y-slab views and
+// write vol_y, the rest the x-slab views into vol_x (summed by fan2d_add).
+constexpr int kFanRows = 63;
+
+// start1[(p*SJ + j)*SR1 + r]: the walk-order column whose extent holds row
+// boundary r of slab j (the first column whose upper edge lies above r), for
+// r = 0 .. NR; 0xFFFF where every column ends at or below r (back_start_kernel
+// at one-row resolution, including the slab's far edge r = NR).
+__global__ void __launch_bounds__(256) fan2d_start_kernel(DevGeom g, const ColRec* __restrict__ rec,
+                                                          unsigned short* __restrict__ start1, int SJ, int SR1)
+{
+    const long idx = blockIdx.x * 256L + threadIdx.x;
+    if (idx >= (long)g.PN * SJ * SR1) return;
+    const int r = (int)(idx % SR1);
+    const long pj = idx / SR1;
+    const int j = (int)(pj % SJ);
+    const int p = (int)(pj / SJ);
+    const ViewParams vp = g.vp[p];
+    const int DNU = g.DNU;
+    if (j >= vp.NS || r > vp.NR) { start1[idx] = 0xFFFF; return; }
+    const double rs = fabs((vp.s0 + j * g.dx) / g.dx);
+    const ColRec* R = rec + (long)p * (DNU + 1);
+    auto A = [&](int q) { return vp.rbase + rs * (q < DNU ? R[q].klo : R[DNU - 1].khi); };
+    const double rr = (double)r;
+    if (A(DNU) <= rr) { start1[idx] = 0xFFFF; return; }
+    int lo = 1, hi = DNU;
+    while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if (A(mid) > rr) hi = mid; else lo = mid + 1;
+    }
+    start1[idx] = (unsigned short)(lo - 1);
+}
+
+// Q[p*(DNU + 1) + c] = sum of y * L over walk-order columns below c (c = 0 ..
+// DNU), L as prjprefix_kernel's; one wavefront per view, 64 columns per pass
+template <typename T>
+__global__ void __launch_bounds__(256) fan2d_colprefix_kernel(DevGeom g, const float* __restrict__ prj,
+                                                              T* __restrict__ Q)
+{
+    const int p = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
+    if (p >= g.PN) return;
+    const int DNU = g.DNU, asc = g.vp[p].asc;
+    const double ez = g.ez[0];
+    T* Qv = Q + (long)p * (DNU + 1);
+    if (lane == 0) Qv[0] = T(0);
+    T carry = T(0);
+    for (int c0 = 0; c0 < DNU; c0 += 64) {
+        const int c = c0 + lane;
+        T v = T(0);
+        if (c < DNU) {
+            const long cu = (long)p * DNU + (asc ? c : DNU - 1 - c);
+            const double L = g.dx * sqrt(g.exy2[cu] + ez * ez) * fabs(g.tau[cu]);
+            v = (T)((double)prj[cu] * L);
+        }
+        const T r = wave_inclusive_scan(v) + carry;
+        if (c < DNU) Qv[c + 1] = r;
+        carry = readlane_t(r, 63);
+    }
+}
 
 template <typename T>
-__global__ void __launch_bounds__(256) fan2d_back_kernel(DevGeom g, const T* __restrict__ P, float* __restrict__ vol_y,
+__global__ void __launch_bounds__(256) fan2d_back_kernel(DevGeom g, const T* __restrict__ Q, float* __restrict__ vol_y,
                                                          float* __restrict__ vol_x, const int* __restrict__ views_y,
                                                          int ny, const int* __restrict__ views_x, int nx, int nby,
-                                                         int ntr_y, int ntr_x, const ColRec* __restrict__ rec,
-                                                         const unsigned short* __restrict__ start, int SJ, int SR)
+                                                         int ntr_y, int ntr_x,
+                                                         const unsigned short* __restrict__ start1, int SJ, int SR1)
 {
-    constexpr int NM = kFanRows;
+    __shared__ double sacc[4][64];
     const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int lane = threadIdx.x & 63;
     const int ys = blockIdx.x < (unsigned)nby;
     const int b = ys ? blockIdx.x : blockIdx.x - nby;
-    const int NS = ys ? g.YN : g.XN, NR = ys ? g.XN : g.YN, ntr = ys ? ntr_y : ntr_x;
+    const int NR = ys ? g.XN : g.YN, ntr = ys ? ntr_y : ntr_x;
     const int rt = b % ntr;
-    const int j = (b / ntr) * 4 + wv;
-    if (j >= NS) return;
+    const int j = b / ntr;
     const int* __restrict__ views = ys ? views_y : views_x;
     const int nviews = ys ? ny : nx;
     float* __restrict__ vol = ys ? vol_y : vol_x;
-    const int r0 = rt * NM;
-    const int nm = min(NM, NR - r0);
+    const int r0 = rt * kFanRows;
+    const int nm = min(kFanRows, NR - r0);
+    const int r = min(r0 + lane, NR);            // this lane's row boundary (lanes past the tile repeat the last)
     const int DNU = g.DNU;
-    const long srow = (long)j * SR + (r0 >> 3);
-    const long sstride = (long)SJ * SR;
-    double acc[NM];
-#pragma unroll
-    for (int i = 0; i < NM; ++i) acc[i] = 0.0;
-
-    for (int base = 0; base < nviews; base += 64) {
-        const int iv = base + lane;
-        const int p = iv < nviews ? views[iv] : views[0];
-        const int st = iv < nviews ? (int)start[p * sstride + srow] : 0xFFFF;
-        bool act = st != 0xFFFF;
-        const int stc = act ? st : 0;
+    double acc = 0.0;
+#pragma unroll 2
+    for (int iv = wv; iv < nviews; iv += 4) {
+        const int p = __builtin_amdgcn_readfirstlane(views[iv]);
         const ViewParams vp = g.vp[p];
-        const double inv_sj = g.invs[(long)p * g.sjs + j];
-        const double inv_rs = fabs(g.dx * inv_sj);
-        // row boundary r0 + i in slope space, km = (r0 + i - rbase) / |rs|
-        double km = ((double)r0 - vp.rbase) * inv_rs;
-        const ColRec* __restrict__ rq = rec + (long)p * (DNU + 1) + stc;
-        const ColRec r = rq[0];
-        act = act && !(r.klo >= fma((double)nm, inv_rs, km));
-        // a view's v-prefix entries in walk order (one row: the entry is y * L itself);
-        // the sentinel column past either end reads 0
-        const T* __restrict__ Pv = P + (long)p * DNU;
+        const double inv_rs = fabs(g.dx * g.invs[(long)p * g.sjs + j]);
+        const double km = ((double)r - vp.rbase) * inv_rs;
+        const int t0 = start1[((long)p * SJ + j) * SR1 + r];
+        const int t = t0 == 0xFFFF ? DNU : t0;
+        // column t's lower slope (times sign(s0)) and 1/|slope step|, the
+        // record's values, read from the natural-order tables (8 bytes per
+        // column: neighbouring lanes' columns are a few entries apart); past
+        // the last column (t = DNU, every column below r) the fraction is 0
+        const double sgn_s = vp.s0 > 0.0 ? 1.0 : -1.0;
         const int asc = vp.asc;
-        auto pcol = [&](int c) { return c < DNU ? Pv[asc ? c : DNU - 1 - c] : T(0); };
-        double klo = r.klo, khi = r.khi, iw = r.invdk;
-        int c = stc;
-        T D = pcol(c);
-        T Dn = pcol(c + 1);
-        double rn_khi = rq[1].khi, rn_iw = rq[1].invdk;
-        T Tsum = T(0), Wprev = T(0);
-#pragma unroll
-        for (int i = 0; i <= NM; ++i) {
-            if (i <= nm) {
-                if (act) {
-                    // advance while the column ends at or below this row boundary;
-                    // the sentinel's NaN upper edge stops the walk
-                    while (km >= khi) {
-                        Tsum += D;
-                        klo = khi;
-                        khi = rn_khi;
-                        iw = rn_iw;
-                        D = Dn;
-                        ++c;
-                        rn_khi = rq[c + 1 - stc].khi;
-                        rn_iw = rq[c + 1 - stc].invdk;
-                        Dn = pcol(c + 1);
-                    }
-                    const double fr = fmin(fmax((km - klo) * iw, 0.0), 1.0);
-                    km += inv_rs;
-                    const T W = Tsum + (T)fr * D;
-                    if (i > 0) acc[i - 1] += (double)(W - Wprev);
-                    Wprev = W;
-                }
-            }
-        }
+        const double klo = sgn_s * g.kap[(long)p * (DNU + 1) + (asc ? t : DNU - t)];
+        const double iw = t < DNU ? g.invdk[(long)p * DNU + (asc ? t : DNU - 1 - t)] : 0.0;
+        const double fr = fmin(fmax((km - klo) * iw, 0.0), 1.0);
+        const T* __restrict__ Qv = Q + (long)p * (DNU + 1);
+        const T q0 = Qv[t], q1 = Qv[min(t + 1, DNU)];
+        const double W = (double)q0 + fr * (double)(q1 - q0);
+        acc += shift_down1(W) - W;                // lane l: W(r + 1) - W(r)
     }
-    // rows' sums over the views (lane order), then lane i stores row i
-    const double sgn = g.dzh < 0.0 ? -1.0 : 1.0;
-    double mine = 0.0;
-#pragma unroll
-    for (int i = 0; i < NM; ++i) {
-        const double tot = readlane_d(wave_inclusive_scan(acc[i]), 63);
-        if (lane == i) mine = tot;
-    }
+    sacc[wv][lane] = acc;
+    __syncthreads();
+    if (wv != 0) return;
+    acc = ((sacc[0][lane] + sacc[1][lane]) + sacc[2][lane]) + sacc[3][lane];
     if (lane < nm) {
+        const double sgn = g.dzh < 0.0 ? -1.0 : 1.0;
         const int m = r0 + lane;
         const long col = ys ? ((long)j * g.XN + m) : ((long)m * g.XN + j);
         const float mk = (g.mask == nullptr || g.mask[col]) ? 1.0f : 0.0f;
-        vol[col] = (float)(mine * sgn) * mk;
+        vol[col] = (float)(acc * sgn) * mk;
     }
 }
 
@@ -231,27 +261,36 @@ hipError_t launch_fan2d_proj(const DevGeom& g, const float* d_vol, void* d_cz, f
 }
 
 template <typename T>
-hipError_t launch_fan2d_back(const DevGeom& g, const T* P, float* d_vol, float* d_scratch, const int* d_views_y,
-                             int ny, const int* d_views_x, int nx, const void* d_rec, const unsigned short* d_start,
-                             int SJ, int SR, hipStream_t s)
+hipError_t launch_fan2d_back(const DevGeom& g, const float* d_prj, void* d_Q, float* d_vol, float* d_scratch,
+                             const int* d_views_y, int ny, const int* d_views_x, int nx,
+                             const unsigned short* d_start1, int SJ, int SR1, hipStream_t s)
 {
+    T* Q = static_cast<T*>(d_Q);
+    hipLaunchKernelGGL(fan2d_colprefix_kernel<T>, dim3((unsigned)((g.PN + 3) / 4)), dim3(256), 0, s, g, d_prj, Q);
     const int ntr_y = (g.XN + kFanRows - 1) / kFanRows, ntr_x = (g.YN + kFanRows - 1) / kFanRows;
-    const int nby = ntr_y * ((g.YN + 3) / 4), nbx = ntr_x * ((g.XN + 3) / 4);
-    hipLaunchKernelGGL(fan2d_back_kernel<T>, dim3((unsigned)(nby + nbx)), dim3(256), 0, s, g, P, d_vol, d_scratch,
-                       d_views_y, ny, d_views_x, nx, nby, ntr_y, ntr_x, (const ColRec*)d_rec, d_start, SJ, SR);
+    const int nby = ntr_y * g.YN, nbx = ntr_x * g.XN;
+    hipLaunchKernelGGL(fan2d_back_kernel<T>, dim3((unsigned)(nby + nbx)), dim3(256), 0, s, g, Q, d_vol, d_scratch,
+                       d_views_y, ny, d_views_x, nx, nby, ntr_y, ntr_x, d_start1, SJ, SR1);
     const long nvox = (long)g.XN * g.YN;
     hipLaunchKernelGGL(fan2d_add, dim3((unsigned)std::min<long>((nvox + 255) / 256, 8192)), dim3(256), 0, s, d_vol,
                        d_scratch, nvox);
     return hipGetLastError();
 }
 
+hipError_t launch_fan2d_start(const DevGeom& g, const void* d_rec, unsigned short* d_start1, int SJ, int SR1,
+                              hipStream_t s)
+{
+    const long n = (long)g.PN * SJ * SR1;
+    hipLaunchKernelGGL(fan2d_start_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, g,
+                       (const ColRec*)d_rec, d_start1, SJ, SR1);
+    return hipGetLastError();
+}
+
 template hipError_t launch_fan2d_proj<double>(const DevGeom&, const float*, void*, float*, hipStream_t);
 template hipError_t launch_fan2d_proj<float>(const DevGeom&, const float*, void*, float*, hipStream_t);
-template hipError_t launch_fan2d_back<double>(const DevGeom&, const double*, float*, float*, const int*, int,
-                                              const int*, int, const void*, const unsigned short*, int, int,
-                                              hipStream_t);
-template hipError_t launch_fan2d_back<float>(const DevGeom&, const float*, float*, float*, const int*, int,
-                                             const int*, int, const void*, const unsigned short*, int, int,
-                                             hipStream_t);
+template hipError_t launch_fan2d_back<double>(const DevGeom&, const float*, void*, float*, float*, const int*, int,
+                                              const int*, int, const unsigned short*, int, int, hipStream_t);
+template hipError_t launch_fan2d_back<float>(const DevGeom&, const float*, void*, float*, float*, const int*, int,
+                                             const int*, int, const unsigned short*, int, int, hipStream_t);
 
 }  // namespace ctlab

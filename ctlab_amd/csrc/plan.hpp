@@ -40,14 +40,20 @@ inline size_t cz_elems(const GeomIn& in)
     return std::max(vox, (size_t)in.YN * (in.XN + 1) + (size_t)in.XN * (in.YN + 1));
 }
 
-// single-slice fan beam (ZN = DNV = 1): lanes over boundaries and views
+// single-slice fan beam (ZN = DNV = 1): lanes over detector boundaries
+// (projector) and over row boundaries (backprojector)
 bool is_fan2d(const DevGeom& g);
+inline bool is_fan2d(const GeomIn& in) { return !in.cone && in.ZN == 1 && in.DNV == 1; }
 template <typename T>
 hipError_t launch_fan2d_proj(const DevGeom& g, const float* d_vol, void* d_cz, float* d_prj, hipStream_t s);
+// d_Q: PN * (DNU + 1) elements of work space (the columns' prefix per view)
 template <typename T>
-hipError_t launch_fan2d_back(const DevGeom& g, const T* P, float* d_vol, float* d_scratch, const int* d_views_y,
-                             int ny, const int* d_views_x, int nx, const void* d_rec, const unsigned short* d_start,
-                             int SJ, int SR, hipStream_t s);
+hipError_t launch_fan2d_back(const DevGeom& g, const float* d_prj, void* d_Q, float* d_vol, float* d_scratch,
+                             const int* d_views_y, int ny, const int* d_views_x, int nx,
+                             const unsigned short* d_start1, int SJ, int SR1, hipStream_t s);
+// walk-order column containing each row boundary r = 0 .. SR1 - 1 of each (view, slab)
+hipError_t launch_fan2d_start(const DevGeom& g, const void* d_rec, unsigned short* d_start1, int SJ, int SR1,
+                              hipStream_t s);
 
 
 template <typename T>
@@ -85,6 +91,8 @@ struct ctlab_plan {
     double* d_rec = nullptr;
     unsigned short* d_start = nullptr;  // PN*SJ*SR first walk-order column per (view, slab, 8 rows)
     int SJ = 0, SR = 0;
+    unsigned short* d_start1 = nullptr; // single-slice fan beam: PN*SJ*SR1, the column per row boundary
+    int SR1 = 0;
     void* d_cz = nullptr;   // z-prefix integral image (nvox elements)
     void* d_P = nullptr;    // projection v-prefix (nprj elements)
     int sat_bits = 64;
