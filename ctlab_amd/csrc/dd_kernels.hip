@@ -244,7 +244,7 @@ static void launch_back_walk(const DevGeom& g, const T* P, float* d_vol, float* 
 template <typename T>
 hipError_t launch_proj(const DevGeom& g, const float* d_vol, void* d_cz, float* d_prj, hipStream_t s)
 {
-    if (is_fan2d(g)) return launch_fan2d_proj<T>(g, d_vol, d_cz, d_prj, s);
+    if (is_fan2d(g) && g.ZN <= kFanProjSlices) return launch_fan2d_proj<T>(g, d_vol, d_cz, d_prj, s);
     T* cz = static_cast<T*>(d_cz);
     const long ncol = (long)g.XN * g.YN;
     hipLaunchKernelGGL(zprefix_kernel<T>, dim3((unsigned)((ncol + 3) / 4)), dim3(256), 0, s,

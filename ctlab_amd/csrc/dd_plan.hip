@@ -316,9 +316,11 @@ static void ws_need(const GeomIn& in, size_t& ncz, size_t& nP)
     GeomIn g;
     g.XN = in.XN; g.YN = in.YN;
     g.ZN = std::min(in.ZN, kBlock);
+    g.DNV = std::min(in.DNV, kBlock);
+    g.cone = in.cone;
     ncz = cz_elems(g);
     nP = (size_t)in.PN * in.DNU * std::min(in.DNV, kBlock);
-    if (is_fan2d(in)) nP = std::max(nP, (size_t)in.PN * (in.DNU + 1));   // the columns' prefix per view
+    if (is_fan2d(in)) nP = std::max(nP, (size_t)in.ZN * in.PN * (in.DNU + 1));   // the columns' prefix
 }
 
 // (Re)allocates the integral-image and v-prefix workspaces for `bits`.  Both

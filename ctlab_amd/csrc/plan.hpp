@@ -29,24 +29,26 @@ struct GeomIn {
 constexpr int kBlock = 64;
 inline bool is_blocked(const GeomIn& in) { return in.ZN > kBlock || in.DNV > kBlock; }
 
+// fan beam with at most kFanSlices slices (ZN = DNV): lanes over detector
+// boundaries (projector) and over row boundaries (backprojector)
+constexpr int kFanSlices = 16;
+constexpr int kFanProjSlices = 8;   // the projector's bound (see fan2d.hip)
+bool is_fan2d(const DevGeom& g);
+inline bool is_fan2d(const GeomIn& in) { return !in.cone && in.ZN == in.DNV && in.ZN <= kFanSlices; }
+
 // integral-image elements: one per voxel (the projector's row reads stay on
-// the rows of the slab it walks); a single slice also holds the row prefixes
-// of both slab orientations (YN * (XN + 1) + XN * (YN + 1) entries) that the
-// single-slice fan-beam projector reads
+// the rows of the slab it walks); a fan beam of few slices holds instead the
+// row prefixes of both slab orientations per slice (YN * (XN + 1) + XN *
+// (YN + 1) entries each)
 inline size_t cz_elems(const GeomIn& in)
 {
     const size_t vox = (size_t)in.XN * in.YN * in.ZN;
-    if (in.ZN != 1) return vox;
-    return std::max(vox, (size_t)in.YN * (in.XN + 1) + (size_t)in.XN * (in.YN + 1));
+    if (!is_fan2d(in)) return vox;
+    return std::max(vox, in.ZN * ((size_t)in.YN * (in.XN + 1) + (size_t)in.XN * (in.YN + 1)));
 }
-
-// single-slice fan beam (ZN = DNV = 1): lanes over detector boundaries
-// (projector) and over row boundaries (backprojector)
-bool is_fan2d(const DevGeom& g);
-inline bool is_fan2d(const GeomIn& in) { return !in.cone && in.ZN == 1 && in.DNV == 1; }
 template <typename T>
 hipError_t launch_fan2d_proj(const DevGeom& g, const float* d_vol, void* d_cz, float* d_prj, hipStream_t s);
-// d_Q: PN * (DNU + 1) elements of work space (the columns' prefix per view)
+// d_Q: ZN * PN * (DNU + 1) elements of work space (the columns' prefix per slice and view)
 template <typename T>
 hipError_t launch_fan2d_back(const DevGeom& g, const float* d_prj, void* d_Q, float* d_vol, float* d_scratch,
                              const int* d_views_y, int ny, const int* d_views_x, int nx,

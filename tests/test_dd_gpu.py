@@ -215,29 +215,32 @@ def test_cfg1_fan_beam_parity(cuda):
     assert rel_l2(dd.DD2_multiGPU(g, prj, 2), O.back(g, prj)) <= TOL64
 
 
-# The single-slice fan beam has kernels of its own (lanes over detector
-# boundaries and over views): odd image sizes, detectors narrower than one
-# 63-column tile and spanning several, flat and arc cells, masks, view counts
-# below and above 64 per slab orientation (one view: one orientation empty),
-# row tiles cut by the image edge and views on axes and 45-degree ties; both
-# integral-image precisions and the adjoint identity.
+# Fan beams of up to 16 slices have kernels of their own (lanes over detector
+# boundaries and over row boundaries): odd image sizes, detectors narrower
+# than one 63-column tile and spanning several, flat and arc cells, masks,
+# view counts below and above 64 per slab orientation (one view: one
+# orientation empty), row tiles cut by the image edge, views on axes and
+# 45-degree ties, and 1, 3, 5 and 16 slices; both integral-image precisions
+# and the adjoint identity.
 FAN2D_CASES = [
-    # XN, YN, PN, DNU, arc, masked, ang0
-    (64, 64, 60, 140, True, False, 0.02),
-    (37, 53, 7, 40, False, True, 0.3),
-    (90, 71, 200, 191, True, True, 0.0),
-    (20, 20, 1, 63, False, False, 0.7),
+    # XN, YN, SLN, PN, DNU, arc, masked, ang0
+    (64, 64, 1, 60, 140, True, False, 0.02),
+    (37, 53, 3, 7, 40, False, True, 0.3),
+    (90, 71, 1, 200, 191, True, True, 0.0),
+    (20, 20, 1, 1, 63, False, False, 0.7),
+    (48, 40, 5, 30, 70, True, False, 0.1),
+    (33, 29, 16, 90, 66, True, True, 0.4),
 ]
 
 
-@pytest.mark.parametrize("XN,YN,PN,DNU,arc,masked,ang0", FAN2D_CASES)
-def test_fan2d_parity(cuda, XN, YN, PN, DNU, arc, masked, ang0):
-    print(f"case fan2d XN={XN} YN={YN} PN={PN} DNU={DNU} arc={arc} mask={masked}", flush=True)
+@pytest.mark.parametrize("XN,YN,SLN,PN,DNU,arc,masked,ang0", FAN2D_CASES)
+def test_fan2d_parity(cuda, XN, YN, SLN, PN, DNU, arc, masked, ang0):
+    print(f"case fan2d XN={XN} YN={YN} SLN={SLN} PN={PN} DNU={DNU} arc={arc} mask={masked}", flush=True)
     mask = None
     if masked:
         yy, xx = np.mgrid[0:YN, 0:XN]
         mask = ((xx - XN / 2) ** 2 + (yy - YN / 2) ** 2 <= (0.45 * min(XN, YN)) ** 2).astype(np.uint8)
-    g = G.fan_multislice(XN=XN, YN=YN, SLN=1, dx=1.0, PN=PN, DNU=DNU, col_size=1.1, sid=150.0, sdd=260.0,
+    g = G.fan_multislice(XN=XN, YN=YN, SLN=SLN, dx=1.0, PN=PN, DNU=DNU, col_size=1.1, sid=150.0, sdd=260.0,
                          arc=arc, mask=mask, ang0=ang0)
     rng = np.random.default_rng(XN + PN)
     img = rng.random(g.vol_shape).astype(np.float32)
@@ -256,7 +259,7 @@ def test_fan2d_parity(cuda, XN, YN, PN, DNU, arc, masked, ang0):
     assert e_p32 <= TOL32 and e_b32 <= TOL32
     assert abs(lhs - rhs) / abs(lhs) < 1e-6
     if masked:
-        assert np.all(aty.reshape(YN, XN)[mask == 0] == 0)
+        assert np.all(aty.reshape(YN, XN, SLN)[mask == 0] == 0)
 
 
 def test_host_abi_equals_plan(cuda):
