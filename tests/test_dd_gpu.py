@@ -284,6 +284,13 @@ def test_host_abi_chunked_forward_equals_plan(cuda, pn):
     ref = _run(g, vol=vol).reshape(-1)
     for _ in range(2):
         assert np.array_equal(dd.DD3Proj_gpu(g, vol).reshape(-1), ref)
+    # the fan-beam entry point takes the same cached, chunked path
+    for sln in (1, 3):
+        gf = G.fan_multislice(XN=40, YN=36, SLN=sln, dx=1.0, PN=pn, DNU=70, col_size=1.1, sid=150.0, sdd=260.0)
+        img = np.random.default_rng(22 + sln).random(gf.vol_shape).astype(np.float32)
+        reff = _run(gf, vol=img).reshape(-1)
+        for _ in range(2):
+            assert np.array_equal(dd.DD2_multiGPU(gf, img, 0).reshape(-1), reff)
 
 
 def test_gpu_adjoint_identity(cuda):
