@@ -406,8 +406,9 @@ ctlab_plan* finish_plan(ctlab_plan* P, int gpunum, const ctlab_plan* share_ws)
         free_plan(P);
         return nullptr;
     }
-    // single-slice fan beam: the one-row start table of its backprojector, up
-    // to 1 GiB (above that the backprojector takes the general walk)
+    // fan beam of up to kFanSlices slices: the one-row start table of its
+    // backprojector, up to 1 GiB (above that the backprojector takes the
+    // general walk)
     const size_t ns1 = (size_t)P->in.PN * P->SJ * (P->SJ + 1);
     if (is_fan2d(P->in) && ns1 * sizeof(unsigned short) <= ((size_t)1 << 30)) {
         P->SR1 = P->SJ + 1;

@@ -93,7 +93,7 @@ struct ctlab_plan {
     double* d_rec = nullptr;
     unsigned short* d_start = nullptr;  // PN*SJ*SR first walk-order column per (view, slab, 8 rows)
     int SJ = 0, SR = 0;
-    unsigned short* d_start1 = nullptr; // single-slice fan beam: PN*SJ*SR1, the column per row boundary
+    unsigned short* d_start1 = nullptr; // fan beam of few slices: PN*SJ*SR1, the column per row boundary
     int SR1 = 0;
     void* d_cz = nullptr;   // z-prefix integral image (nvox elements)
     void* d_P = nullptr;    // projection v-prefix (nprj elements)
