@@ -32,16 +32,16 @@
 
 namespace ctlab {
 
-// One wavefront per voxel column (lane = z; ZN <= 64): coalesced read of the
-// column, wavefront scan, coalesced write.
+// One wavefront per voxel column (lane = z; ZN <= 64), looping over columns
+// when the grid holds fewer wavefronts: coalesced read of the column,
+// wavefront scan, coalesced write.
 template <typename T>
 __global__ void __launch_bounds__(256)
 zprefix_kernel(const float* __restrict__ vol, T* __restrict__ cz,
                const unsigned char* __restrict__ mask, long ncol, int ZN)
 {
-    const long col = blockIdx.x * 4L + (threadIdx.x >> 6);
-    if (col >= ncol) return;
-    zprefix_wave(gmem(vol), gmem(cz), gmem(mask), col, ZN, threadIdx.x & 63);
+    for (long col = blockIdx.x * 4L + (threadIdx.x >> 6); col < ncol; col += gridDim.x * 4L)
+        zprefix_wave(gmem(vol), gmem(cz), gmem(mask), col, ZN, threadIdx.x & 63);
 }
 
 // Projector walk.  Block = 4 wavefronts = 4 consecutive views of one tile of
@@ -77,29 +77,28 @@ template <typename T>
 __global__ void __launch_bounds__(256)
 prjprefix_kernel(DevGeom g, const float* __restrict__ prj, T* __restrict__ P)
 {
-    const long idx0 = (blockIdx.x * 4L + (threadIdx.x >> 6)) * kPrefixCols;
     const long n = (long)g.PN * g.DNU;
     const int lane = threadIdx.x & 63, DNV = g.DNV;
-    if (idx0 >= n) return;
-    // every load of the wave's rows first (rows past the last read the last one)
     const int vl = min(lane, DNV - 1);
     const double ezv = g.ez[vl];
-    float y[kPrefixCols];
-    double e2[kPrefixCols], ta[kPrefixCols];
+    for (long idx0 = (blockIdx.x * 4L + (threadIdx.x >> 6)) * kPrefixCols; idx0 < n;
+         idx0 += gridDim.x * 4L * kPrefixCols) {
+        // every load of the wave's rows first (rows past the last read the last one)
+        float y[kPrefixCols];
+        double e2[kPrefixCols], ta[kPrefixCols];
 #pragma unroll
-    for (int c = 0; c < kPrefixCols; ++c) {
-        const long idx = min(idx0 + c, n - 1);
-        y[c] = prj[idx * DNV + vl];
-        e2[c] = g.exy2[idx];
-        ta[c] = g.tau[idx];
-    }
+        for (int c = 0; c < kPrefixCols; ++c) {
+            const long idx = min(idx0 + c, n - 1);
+            y[c] = prj[idx * DNV + vl];
+            e2[c] = g.exy2[idx];
+            ta[c] = g.tau[idx];
+        }
 #pragma unroll
-    for (int c = 0; c < kPrefixCols; ++c) {
-        const long idx = idx0 + c;
-        const double L = g.dx * sqrt(e2[c] + ezv * ezv) * fabs(ta[c]);
-        const T r = wave_inclusive_scan(lane < DNV ? (T)((double)y[c] * L) : T(0));
-        if (idx < n && lane < DNV) {
-            P[idx * DNV + lane] = r;
+        for (int c = 0; c < kPrefixCols; ++c) {
+            const long idx = idx0 + c;
+            const double L = g.dx * sqrt(e2[c] + ezv * ezv) * fabs(ta[c]);
+            const T r = wave_inclusive_scan(lane < DNV ? (T)((double)y[c] * L) : T(0));
+            if (idx < n && lane < DNV) P[idx * DNV + lane] = r;
         }
     }
 }
@@ -109,27 +108,28 @@ prjprefix_kernel(DevGeom g, const float* __restrict__ prj, T* __restrict__ P)
 __global__ void __launch_bounds__(256)
 back_start_kernel(DevGeom g, const ColRec* __restrict__ rec, unsigned short* __restrict__ start, int SJ, int SR)
 {
-    const long idx = blockIdx.x * 256L + threadIdx.x;
-    if (idx >= (long)g.PN * SJ * SR) return;
-    const int r8 = (int)(idx % SR);
-    const long pj = idx / SR;
-    const int j = (int)(pj % SJ);
-    const int p = (int)(pj / SJ);
-    const ViewParams vp = g.vp[p];
-    const int DNU = g.DNU;
-    const double r0 = 8.0 * r8;
-    if (j >= vp.NS || r0 >= vp.NR) { start[idx] = 0xFFFF; return; }
-    const double rs = fabs((vp.s0 + j * g.dx) / g.dx);
-    const ColRec* R = rec + (long)p * (DNU + 1);
-    // the records' slopes carry the sign of the source-to-slab distance
-    auto A = [&](int q) { return vp.rbase + rs * (q < DNU ? R[q].klo : R[DNU - 1].khi); };
-    if (A(DNU) <= r0) { start[idx] = 0xFFFF; return; }
-    int lo = 1, hi = DNU;
-    while (lo < hi) {
-        const int mid = (lo + hi) >> 1;
-        if (A(mid) > r0) hi = mid; else lo = mid + 1;
+    const long n = (long)g.PN * SJ * SR;
+    for (long idx = blockIdx.x * 256L + threadIdx.x; idx < n; idx += gridDim.x * 256L) {
+        const int r8 = (int)(idx % SR);
+        const long pj = idx / SR;
+        const int j = (int)(pj % SJ);
+        const int p = (int)(pj / SJ);
+        const ViewParams vp = g.vp[p];
+        const int DNU = g.DNU;
+        const double r0 = 8.0 * r8;
+        if (j >= vp.NS || r0 >= vp.NR) { start[idx] = 0xFFFF; continue; }
+        const double rs = fabs((vp.s0 + j * g.dx) / g.dx);
+        const ColRec* R = rec + (long)p * (DNU + 1);
+        // the records' slopes carry the sign of the source-to-slab distance
+        auto A = [&](int q) { return vp.rbase + rs * (q < DNU ? R[q].klo : R[DNU - 1].khi); };
+        if (A(DNU) <= r0) { start[idx] = 0xFFFF; continue; }
+        int lo = 1, hi = DNU;
+        while (lo < hi) {
+            const int mid = (lo + hi) >> 1;
+            if (A(mid) > r0) hi = mid; else lo = mid + 1;
+        }
+        start[idx] = (unsigned short)(lo - 1);
     }
-    start[idx] = (unsigned short)(lo - 1);
 }
 
 // Block = 4 wavefronts = 4 consecutive slabs of one tile of NM rows (their
@@ -207,6 +207,16 @@ __global__ void __launch_bounds__(256) rows_scatter(const float* __restrict__ sr
 // ---------------------------------------------------------------------------
 static unsigned stream_grid(long n) { return (unsigned)std::max<long>(1, std::min<long>((n + 255) / 256, 8192)); }
 
+// Grids of the kernels that loop over their items (one item per thread, or per
+// wavefront: 4 per block): every item's block when there are few enough,
+// else kMaxBlocks.  A launch holds at most 2^32 work-items, which a volume of
+// 8192 x 8192 columns reaches with one wavefront per column.
+constexpr long kMaxBlocks = 1L << 20;
+static unsigned item_grid(long n, int per_block)
+{
+    return (unsigned)std::max<long>(1, std::min<long>((n + per_block - 1) / per_block, kMaxBlocks));
+}
+
 hipError_t launch_rows_gather(const float* src, float* dst, long nrows, int ns, int off, int nd, hipStream_t s)
 {
     hipLaunchKernelGGL(rows_gather, dim3(stream_grid(nrows * nd)), dim3(256), 0, s, src, dst, nrows, ns, off, nd);
@@ -247,7 +257,7 @@ hipError_t launch_proj(const DevGeom& g, const float* d_vol, void* d_cz, float* 
     if (is_fan2d(g) && g.ZN <= kFanProjSlices) return launch_fan2d_proj<T>(g, d_vol, d_cz, d_prj, s);
     T* cz = static_cast<T*>(d_cz);
     const long ncol = (long)g.XN * g.YN;
-    hipLaunchKernelGGL(zprefix_kernel<T>, dim3((unsigned)((ncol + 3) / 4)), dim3(256), 0, s,
+    hipLaunchKernelGGL(zprefix_kernel<T>, dim3(item_grid(ncol, 4)), dim3(256), 0, s,
                        d_vol, cz, (const unsigned char*)g.mask, ncol, g.ZN);
     switch (g.tile_nu) {
     case 8: launch_proj_walk<T, 8>(g, cz, d_prj, s); break;
@@ -261,7 +271,7 @@ hipError_t launch_back_start(const DevGeom& g, const void* d_rec, unsigned short
                              hipStream_t s)
 {
     const long n = (long)g.PN * SJ * SR;
-    hipLaunchKernelGGL(back_start_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, g,
+    hipLaunchKernelGGL(back_start_kernel, dim3(item_grid(n, 256)), dim3(256), 0, s, g,
                        (const ColRec*)d_rec, d_start, SJ, SR);
     return hipGetLastError();
 }
@@ -274,7 +284,7 @@ hipError_t launch_back(const DevGeom& g, const float* d_prj, void* d_P, float* d
     const ColRec* rec = (const ColRec*)d_rec;
     T* P = static_cast<T*>(d_P);
     const long ncu = (long)g.PN * g.DNU;
-    hipLaunchKernelGGL(prjprefix_kernel<T>, dim3((unsigned)((ncu + 4 * kPrefixCols - 1) / (4 * kPrefixCols))), dim3(256), 0, s,
+    hipLaunchKernelGGL(prjprefix_kernel<T>, dim3(item_grid(ncu, 4 * kPrefixCols)), dim3(256), 0, s,
                        g, d_prj, P);
     // y-slab views (slab = y, rows along x) into d_vol, x-slab views into d_scratch, one launch
     switch (g.tile_nm) {

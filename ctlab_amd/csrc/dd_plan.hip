@@ -45,11 +45,16 @@ void clear_error()
     t_msg[0] = 0;
 }
 
+// A failed call is recorded here and taken off the runtime's per-thread last
+// error, which successful calls leave in place: a later launch check
+// (hipGetLastError, here or in the caller's own code) would otherwise report
+// this failure as its own.
 int hip_check(hipError_t e, const char* what)
 {
     if (e == hipSuccess) return 0;
     set_error(e == hipErrorOutOfMemory ? CTLAB_ERR_NOMEM : CTLAB_ERR_HIP, "%s: %s", what,
               hipGetErrorString(e));
+    (void)hipGetLastError();
     return -1;
 }
 
@@ -376,6 +381,25 @@ ctlab_plan* finish_plan(ctlab_plan* P, int gpunum, const ctlab_plan* share_ws)
 {
     if (is_blocked(P->in)) return finish_blocked(P, gpunum, share_ws);
     if (build_tables(P)) { delete P; return nullptr; }
+    {
+        // A launch holds at most 2^32 work-items (256 per block here).  The
+        // walks have one block per 4 views and tile of columns (projector) and
+        // per 4 slabs and tile of rows (backprojector); at the smallest tiles
+        // (8 columns, 8 rows) their grids are the largest any plan launches.
+        const GeomIn& in = P->in;
+        const long lim = 0xFFFFFFFFL / 256;
+        const long g8 = ((in.PN + 3L) / 4 + 7) / 8;
+        const long proj_blocks = 8 * g8 * ((in.DNU + 7L) / 8);
+        const long back_blocks = ((in.XN + 7L) / 8) * ((in.YN + 3L) / 4) + ((in.YN + 7L) / 8) * ((in.XN + 3L) / 4);
+        if (proj_blocks > lim || back_blocks > lim) {
+            set_error(CTLAB_ERR_GEOMETRY,
+                      "%ld projector / %ld backprojector blocks at 8-column and 8-row tiles: more than one launch "
+                      "holds (%ld); fewer views x columns or a smaller XN x YN",
+                      proj_blocks, back_blocks, lim);
+            delete P;
+            return nullptr;
+        }
+    }
     int ndev = 0;
     if (hip_check(hipGetDeviceCount(&ndev), "hipGetDeviceCount")) { delete P; return nullptr; }
     if (gpunum < 0 || gpunum >= ndev) {
@@ -846,18 +870,37 @@ static std::shared_ptr<CacheEntry> cache_get(std::vector<unsigned char>&& key, i
         lock = std::unique_lock<std::mutex>(e->busy);
         return e;
     }
-    e = std::make_shared<CacheEntry>();
+    // a new entry; a partly built one frees what it holds when dropped
+    auto build = [&]() -> std::shared_ptr<CacheEntry> {
+        auto n = std::make_shared<CacheEntry>();
+        n->device = device;
+        n->plan = make();
+        if (!n->plan) return nullptr;
+        DeviceGuard dg(device);
+        const size_t nvox = (size_t)n->plan->in.XN * n->plan->in.YN * n->plan->in.ZN;
+        const size_t nprj = (size_t)n->plan->in.PN * n->plan->in.DNU * n->plan->in.DNV;
+        if (hip_check(hipStreamCreateWithFlags(&n->s, hipStreamNonBlocking), "hipStreamCreate") ||
+            hip_check(hipMalloc(&n->dv, nvox * sizeof(float)), "hipMalloc(volume)") ||
+            hip_check(hipMalloc(&n->dp, nprj * sizeof(float)), "hipMalloc(projection)"))
+            return nullptr;
+        return n;
+    };
+    e = build();
+    if (!e && t_err == CTLAB_ERR_NOMEM) {
+        // Device memory ran out: the cached entries may hold it.  Take them
+        // all out of the cache (an entry another thread is using is freed when
+        // that call ends) and try once more.
+        std::list<std::shared_ptr<CacheEntry>> dropped;
+        {
+            std::lock_guard<std::mutex> g(C.mu);
+            dropped.swap(C.lru);
+        }
+        dropped.clear();
+        clear_error();
+        e = build();
+    }
+    if (!e) return nullptr;
     e->key = std::move(key);
-    e->device = device;
-    e->plan = make();
-    if (!e->plan) return nullptr;
-    DeviceGuard dg(device);
-    const size_t nvox = (size_t)e->plan->in.XN * e->plan->in.YN * e->plan->in.ZN;
-    const size_t nprj = (size_t)e->plan->in.PN * e->plan->in.DNU * e->plan->in.DNV;
-    if (hip_check(hipStreamCreateWithFlags(&e->s, hipStreamNonBlocking), "hipStreamCreate") ||
-        hip_check(hipMalloc(&e->dv, nvox * sizeof(float)), "hipMalloc(volume)") ||
-        hip_check(hipMalloc(&e->dp, nprj * sizeof(float)), "hipMalloc(projection)"))
-        return nullptr;
     lock = std::unique_lock<std::mutex>(e->busy);
     std::vector<std::shared_ptr<CacheEntry>> dropped;   // freed after the cache lock is released
     {

@@ -98,7 +98,7 @@ enum {
     CTLAB_OK = 0,
     CTLAB_ERR_ARG = 1,       /* invalid argument (sizes, NULL, non-finite geometry) */
     CTLAB_ERR_MODE = 2,      /* unsupported prjMode / method */
-    CTLAB_ERR_GEOMETRY = 3,  /* unsupported geometry (source inside volume, ZN or DNV > 64, ...) */
+    CTLAB_ERR_GEOMETRY = 3,  /* unsupported geometry (source inside volume, uneven rows, ...) */
     CTLAB_ERR_HIP = 4,       /* HIP runtime error */
     CTLAB_ERR_NOMEM = 5,     /* device allocation failed */
     CTLAB_ERR_IO = 6         /* file could not be opened, read or written in full */

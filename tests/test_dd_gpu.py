@@ -293,6 +293,95 @@ def test_host_abi_chunked_forward_equals_plan(cuda, pn):
             assert np.array_equal(dd.DD2_multiGPU(gf, img, 0).reshape(-1), reff)
 
 
+def test_failed_device_allocation_leaves_no_stale_error(cuda):
+    """A plan whose backprojector start table (2 bytes per view, slab and
+    8-row group) is larger than the device's memory fails with
+    CTLAB_ERR_NOMEM and leaves nothing behind: the failed allocation is not
+    the runtime's last error afterwards, so the next plan's launch checks and
+    torch's own pass, with the same result as before."""
+    import torch
+    from ctlab_amd import _lib
+    g = _cone(PN=12)
+    vol = _phantom(g, 31)
+    ref = _run(g, vol=vol)
+    total = torch.cuda.get_device_properties(0).total_memory
+    side = 16000
+    per_view = side * ((side + 7) // 8 + 1) * 2
+    pn = int(np.ceil(1.2 * total / per_view))
+    big = G.circular_cone(XN=side, YN=side, ZN=2, dx=100.0 / side, dz=1.0, PN=pn, DNU=16, DNV=2,
+                          col_size=1.3, row_size=1.8, sid=150.0, sdd=260.0)
+    print(f"plan of {side}x{side}x2, {pn} views: {pn * per_view / 2**30:.0f} GiB start table", flush=True)
+    with pytest.raises(_lib.CtlabError) as ei:
+        dd.Plan(big)
+    assert ei.value.code == 5   # CTLAB_ERR_NOMEM
+    assert np.array_equal(_run(g, vol=vol), ref)
+    assert float(torch.ones(4096, device="cuda").sum()) == 4096.0
+
+
+def test_host_cache_drops_entries_when_device_memory_runs_out(cuda):
+    """Cached host-ABI entries hold device memory (tables, start tables,
+    buffers).  When a new geometry's allocation fails, the cache drops its
+    entries and the call builds its plan again: a caller cycling through more
+    geometries than the device holds sees no error, and the result equals a
+    call on an empty cache.  Entries are 8192 x 8192 x 2 volumes with 512
+    views, sized from the measured free memory so that a few more calls than
+    fit are made."""
+    import torch
+    from ctlab_amd import _lib
+    L = _lib.lib()
+    torch.cuda.empty_cache()
+    prev = L.ctlab_plan_cache_set_limit(64)
+
+    def geom(k):
+        return G.circular_cone(XN=8192, YN=8192, ZN=2, dx=0.015, dz=1.0, PN=512, DNU=16, DNV=2, col_size=1.3,
+                               row_size=1.8, sid=150.0, sdd=260.0, ang0=0.01 * k)
+
+    try:
+        L.ctlab_plan_cache_clear()
+        vol = np.full(8192 * 8192 * 2, 0.5, np.float32)
+        free0 = torch.cuda.mem_get_info()[0]
+        out = dd.DD3Proj_gpu(geom(0), vol)
+        per = free0 - torch.cuda.mem_get_info()[0]
+        assert per > 0
+        n = min(int(free0 // per) + 3, 40)
+        print(f"cache entry {per / 2**30:.1f} GiB of {free0 / 2**30:.0f} GiB free: {n} calls", flush=True)
+        for k in range(1, n):
+            out = dd.DD3Proj_gpu(geom(k), vol)
+            print(f"call {k}: {L.ctlab_plan_cache_size()} entries", flush=True)
+        assert L.ctlab_plan_cache_size() < n   # entries were dropped on the way
+        assert np.isfinite(out).all() and np.abs(out).max() > 0
+        L.ctlab_plan_cache_clear()
+        assert np.array_equal(dd.DD3Proj_gpu(geom(n - 1), vol), out)
+    finally:
+        L.ctlab_plan_cache_set_limit(prev)
+        L.ctlab_plan_cache_clear()
+
+
+def test_grids_past_the_block_cap(cuda):
+    """The z-prefix and start-table kernels loop over their items when the grid
+    is capped (2^20 blocks): 2100 x 2100 columns are more wavefronts than that
+    grid holds, and 500 views x 2100 slabs x 264 row groups more start-table
+    entries.  The projector matches the oracle; the backprojector, whose last
+    views' start entries come from the loop's second pass, passes the adjoint
+    identity."""
+    import torch
+    g = G.circular_cone(XN=2100, YN=2100, ZN=2, dx=0.05, dz=1.0, PN=500, DNU=32, DNV=2, col_size=8.0,
+                        row_size=1.8, sid=150.0, sdd=260.0, ang0=0.1)
+    assert g.XN * g.YN > 4 * (1 << 20) and g.PN * 2100 * ((2100 + 7) // 8 + 1) > 256 * (1 << 20)
+    rng = np.random.default_rng(41)
+    x = rng.random(g.nvox()).astype(np.float32)
+    y = rng.random(g.nprj()).astype(np.float32)
+    pl = dd.Plan(g)
+    ax = pl.proj(torch.from_numpy(x).cuda()).cpu().numpy().reshape(-1)
+    aty = pl.back(torch.from_numpy(y).cuda()).cpu().numpy().reshape(-1)
+    ref = O.proj(g, x.reshape(g.vol_shape)).reshape(-1)
+    err = rel_l2(ax, ref)
+    lhs, rhs = ax.astype(np.float64) @ y, x.astype(np.float64) @ aty.astype(np.float64)
+    print(f"projector vs oracle {err:.2e}, adjoint mismatch {abs(lhs - rhs) / abs(lhs):.2e}", flush=True)
+    assert err < TOL64
+    assert abs(lhs - rhs) / abs(lhs) < 1e-6
+
+
 def test_gpu_adjoint_identity(cuda):
     g = _cone(PN=64)
     rng = np.random.default_rng(20)
