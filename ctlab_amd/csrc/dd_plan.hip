@@ -889,15 +889,17 @@ static std::shared_ptr<CacheEntry> cache_get(std::vector<unsigned char>&& key, i
     if (!e && t_err == CTLAB_ERR_NOMEM) {
         // Device memory ran out: the cached entries may hold it.  Take them
         // all out of the cache (an entry another thread is using is freed when
-        // that call ends) and try once more.
+        // that call ends) and, if there were any, try once more.
         std::list<std::shared_ptr<CacheEntry>> dropped;
         {
             std::lock_guard<std::mutex> g(C.mu);
             dropped.swap(C.lru);
         }
-        dropped.clear();
-        clear_error();
-        e = build();
+        if (!dropped.empty()) {
+            dropped.clear();
+            clear_error();
+            e = build();
+        }
     }
     if (!e) return nullptr;
     e->key = std::move(key);
